@@ -1,0 +1,172 @@
+"""PBILS throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
+
+A step is one PBILS iteration (select + steepest descent of every individual + record + GWW + perturb + G
+rebuild) of the planted-partition workload: SBM n=10000 in 5 equal hidden clusters, p_in 0.4, p_out 0.05,
+graph-seed 2; k=5, pop 128, gww 0.1, seed 42, early stop off. The graph is uploaded to HBM before the timed region;
+the timed region is one solve of K iterations (its allocation and initial rebuild included). Prints one JSON line.
+
+value       PBILS iterations per second (whole job, 1 GPU)
+roofline    the descent kernel (the dominant one): algorithmic bytes per launch / its mean launch time, measured
+            with HIP events on the library's stream over the timed region
+cpu_baseline  the reference CPU solver (built out of tree into oracle/_ref) on this host's cores, on a bounded
+            sample of the same workload; bit_exact reports that the GPU reproduces its result on that sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c1": dict(workload="ER G(n=1000, p=0.5) graph-seed 1; k=2, pop=128, gww=0.1, seed=42, early stop off",
+               gen=("er", 1000, 0.5, 1), k=2, population=128, gww=0.1, seed=42, iterations=100, cpu_iterations=20),
+    "c2": dict(workload="SBM n=10000, 5 equal blocks, p_in=0.4, p_out=0.05, graph-seed 2; k=5, pop=128, gww=0.1, "
+                        "seed=42, early stop off",
+               gen=("sbm", 10000, 5, 0.4, 0.05, 2), k=5, population=128, gww=0.1, seed=42, iterations=200,
+               cpu_iterations=1),
+}
+
+
+def make_graph(cga, gen):
+    if gen[0] == "er":
+        return cga.erdos_renyi(gen[1], gen[2], gen[3])
+    return cga.sbm(gen[1], gen[2], gen[3], gen[4], gen[5])
+
+
+def descent_bytes_per_move(n, k, descent):
+    """Algorithmic bytes one accepted move streams: the slot's state scanned once (G and labels, or e and the
+    label bit for the k = 2 register kernel) plus the adjacency row of the moved vertex."""
+    words = max(1, (n + 63) // 64)
+    row = 8 * words
+    if descent == "k2-registers":
+        return 4 * n + row
+    gbytes = 2 if n <= 32767 else 4
+    return k * n * gbytes + n + row
+
+
+def cpu_baseline(cfg, graph, threads, iterations):
+    import oracle
+
+    if not oracle.reference_available():
+        return None, None
+    ref = oracle.Reference()
+    og = oracle.Graph(graph.bits, graph.n, graph.edges)
+    t0 = time.perf_counter()
+    res = ref.solve_pbils(og, k=cfg["k"], population=cfg["population"], iterations=iterations, early_stop=iterations,
+                          gww=cfg["gww"], seed=cfg["seed"], threads=threads)
+    wall = time.perf_counter() - t0
+    return res, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import cuda_graph_clustering_amd as cga
+
+    cfg = CONFIGS[args.config]
+    steps = args.steps if args.steps else cfg["iterations"]
+    graph = make_graph(cga, cfg["gen"])
+    common = dict(k=cfg["k"], population=cfg["population"], gww=cfg["gww"], seed=cfg["seed"])
+
+    with cga.DeviceGraph(graph) as dg:
+        if args.warmup > 0:
+            dg.solve_pbils(iterations=args.warmup, early_stop=args.warmup, **common)
+        t0 = time.perf_counter()
+        res = dg.solve_pbils(iterations=steps, early_stop=steps, profile=True, **common)
+        wall = time.perf_counter() - t0
+        sample = None
+        if not args.no_cpu:
+            cpu_it = cfg["cpu_iterations"]
+            t1 = time.perf_counter()
+            gpu_sample = dg.solve_pbils(iterations=cpu_it, early_stop=cpu_it, **common)
+            gpu_sample_wall = time.perf_counter() - t1
+            cpu_res, cpu_wall = cpu_baseline(cfg, graph, args.cpu_threads, cpu_it)
+            if cpu_res is not None:
+                same = (cpu_res["objective"] == gpu_sample["objective"] and
+                        cpu_res["accepted_moves"] == gpu_sample["accepted_moves"] and
+                        (cpu_res["labels"] == gpu_sample["labels"]).all())
+                sample = dict(iterations=cpu_it, cpu_seconds=cpu_res["seconds"], cpu_wall=cpu_wall,
+                              gpu_seconds=gpu_sample["seconds"], gpu_wall=gpu_sample_wall, bit_exact=bool(same))
+
+    n, k = graph.n, cfg["k"]
+    iters = res["iterations_done"]
+    value = iters / wall
+    launches = max(1, res["search_launches"])
+    mean_launch_s = res["search_ms"] / 1e3 / launches
+    moves_per_launch = res["accepted_moves"] / launches
+    bytes_per_launch = moves_per_launch * descent_bytes_per_move(n, k, res["descent_name"])
+    achieved = bytes_per_launch / mean_launch_s / 1e9 if mean_launch_s > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / max(iters, 1), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16" if n <= 32767 else "int32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n": n, "edges": graph.edges, "k": k,
+                   "population": cfg["population"], "iterations": steps, "descent": res["descent_name"],
+                   "descent_threads": res["threads"]},
+        "moves_per_s": round(res["accepted_moves"] / wall, 1),
+        "accepted_moves": res["accepted_moves"],
+        "objective": res["objective"],
+        "device_ms": {"descent": round(res["search_ms"], 3), "rebuild": round(res["rebuild_ms"], 3),
+                      "iterations": round(res["iteration_ms"], 3)},
+        "roofline": {
+            "kernel": "descent (" + res["descent_name"] + ")",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": round(bytes_per_launch),
+            "mean_launch_ms": round(1e3 * mean_launch_s, 4),
+        },
+    }
+    if sample is not None:
+        cpu_rate = sample["iterations"] / sample["cpu_seconds"]
+        out["cpu_baseline"] = {
+            "value": round(cpu_rate, 5),
+            "unit": "iter/s",
+            "cores": args.cpu_threads,
+            "kind": "reference",
+            "sample": f"the first {sample['iterations']} PBILS iteration(s) of the same workload from the random "
+                      f"initial population (init and first descents included), reference SolveCpu with "
+                      f"{args.cpu_threads} threads, its own clock",
+            "gpu_same_sample_iter_per_s": round(sample["iterations"] / sample["gpu_seconds"], 3),
+            "gpu_over_cpu_same_sample": round(sample["cpu_seconds"] / sample["gpu_seconds"], 1),
+            "bit_exact": sample["bit_exact"],
+        }
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

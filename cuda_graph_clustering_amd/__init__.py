@@ -1,0 +1,304 @@
+"""Python binding of the MI355X PBILS correlation-clustering library (lib/libcc_hip.so) over its C ABI.
+
+The library is the product: every solve runs the HIP kernels of csrc/ on the GPU. Importing this module raises if
+the library has not been built (`make -C cuda_graph_clustering_amd`); there is no CPU fallback.
+
+Mirrors the reference's PbilsParams / PbilsResult (include/cc_pbils.hpp:26-59 of the reference) and its GPU entry
+points SolveGpu, SolveNeighborhoodGpu and SolveNeighborhoodWithManyLocalSearchesGpu (include/cc_gpu.hpp:12-20).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcc_hip.so")
+CLI_PATH = os.path.join(HERE, "bin", "cc_gpu")
+
+LS_AUTO, LS_GLOBAL, LS_SHARED, LS_WIDE = 0, 1, 2, 3
+LS_KERNELS = {"auto": LS_AUTO, "global": LS_GLOBAL, "shared": LS_SHARED, "wide": LS_WIDE}
+DESCENTS = {0: "generic-global", 1: "generic-lds", 2: "k2-registers", 3: "k2-wide", 4: "k3-wide"}
+
+OK, EINVAL, ENOMEM, EDEVICE, ENODEV = 0, 1, 2, 3, 4
+
+
+class CcError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_int32),
+        ("population", ctypes.c_int32),
+        ("tournament", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("early_stop", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("perturbation", ctypes.c_double),
+        ("gww", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("time_limit_sec", ctypes.c_double),
+        ("verbose", ctypes.c_int32),
+        ("ls_kernel", ctypes.c_int32),
+        ("profile", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("objective", ctypes.c_int64),
+        ("iterations_done", ctypes.c_int32),
+        ("clusters_used", ctypes.c_int32),
+        ("local_searches", ctypes.c_int64),
+        ("accepted_moves", ctypes.c_int64),
+        ("seconds", ctypes.c_double),
+        ("descent", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("blocks", ctypes.c_int32),
+        ("vertices_per_thread", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int64),
+        ("search_ms", ctypes.c_double),
+        ("rebuild_ms", ctypes.c_double),
+        ("iteration_ms", ctypes.c_double),
+        ("search_launches", ctypes.c_int64),
+        ("rebuild_launches", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["descent_name"] = DESCENTS.get(self.descent, "?")
+        return d
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("compute_units", ctypes.c_int32),
+        ("lds_per_block", ctypes.c_int32),
+        ("l2_bytes", ctypes.c_int32),
+        ("clock_khz", ctypes.c_int32),
+        ("total_bytes", ctypes.c_int64),
+        ("free_bytes", ctypes.c_int64),
+    ]
+
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# Every symbol the C ABI (include/cc_hip.h) exports.
+ABI_SYMBOLS = [
+    "cc_pbils_default_params",
+    "cc_hip_solve_pbils",
+    "cc_hip_solve_neighborhood",
+    "cc_hip_graph_upload",
+    "cc_hip_graph_free",
+    "cc_hip_graph_solve_pbils",
+    "cc_hip_graph_solve_neighborhood",
+    "cc_words64",
+    "cc_graph_erdos_renyi",
+    "cc_graph_sbm",
+    "cc_graph_chung_lu",
+    "cc_objective_direct",
+    "cc_objective_bitwise",
+    "cc_hip_device_info",
+]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `make -C cuda_graph_clustering_amd` (there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.cc_pbils_default_params.argtypes = [ctypes.POINTER(Params)]
+    lib.cc_hip_solve_pbils.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(Params), _i32p,
+                                       ctypes.POINTER(Stats), _i64p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_solve_neighborhood.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(Params),
+                                              ctypes.c_int32, _i32p, ctypes.POINTER(Stats), ctypes.c_char_p,
+                                              ctypes.c_size_t]
+    lib.cc_hip_graph_upload.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_graph_free.argtypes = [ctypes.c_void_p]
+    lib.cc_hip_graph_solve_pbils.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), _i32p, ctypes.POINTER(Stats),
+                                             _i64p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_graph_solve_neighborhood.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, _i32p,
+                                                    ctypes.POINTER(Stats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_words64.argtypes = [ctypes.c_uint32]
+    lib.cc_words64.restype = ctypes.c_uint32
+    lib.cc_graph_erdos_renyi.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, _u64p, ctypes.c_int32]
+    lib.cc_graph_erdos_renyi.restype = ctypes.c_uint64
+    lib.cc_graph_sbm.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                 _u64p, ctypes.c_int32]
+    lib.cc_graph_sbm.restype = ctypes.c_uint64
+    lib.cc_graph_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _u64p,
+                                      ctypes.c_int32]
+    lib.cc_graph_chung_lu.restype = ctypes.c_uint64
+    for name in ("cc_objective_direct", "cc_objective_bitwise"):
+        getattr(lib, name).argtypes = [_u64p, ctypes.c_uint32, _i32p]
+        getattr(lib, name).restype = ctypes.c_int64
+    lib.cc_hip_device_info.argtypes = [ctypes.POINTER(DeviceInfo), ctypes.c_char_p, ctypes.c_size_t]
+    return lib
+
+
+lib = _load()
+
+
+def words64(n):
+    return max(1, (n + 63) // 64)
+
+
+class Graph:
+    """Host bitset: n rows of words64(n) uint64 words."""
+
+    def __init__(self, bits, n, edges):
+        self.n = int(n)
+        self.words = words64(self.n)
+        self.bits = np.ascontiguousarray(bits, dtype=np.uint64).reshape(self.n, self.words)
+        self.edges = int(edges)
+
+    def ptr(self):
+        return self.bits.ctypes.data_as(_u64p)
+
+
+def erdos_renyi(n, p, seed, threads=0):
+    bits = np.zeros((n, words64(n)), dtype=np.uint64)
+    edges = lib.cc_graph_erdos_renyi(n, p, seed, bits.ctypes.data_as(_u64p), threads)
+    return Graph(bits, n, edges)
+
+
+def sbm(n, blocks, p_in, p_out, seed, threads=0):
+    bits = np.zeros((n, words64(n)), dtype=np.uint64)
+    edges = lib.cc_graph_sbm(n, blocks, p_in, p_out, seed, bits.ctypes.data_as(_u64p), threads)
+    return Graph(bits, n, edges)
+
+
+def chung_lu(n, gamma, mean_degree, seed, threads=0):
+    bits = np.zeros((n, words64(n)), dtype=np.uint64)
+    edges = lib.cc_graph_chung_lu(n, gamma, mean_degree, seed, bits.ctypes.data_as(_u64p), threads)
+    return Graph(bits, n, edges)
+
+
+def from_bits(bits, n):
+    bits = np.ascontiguousarray(bits, dtype=np.uint64).reshape(n, words64(n))
+    edges = int(np.unpackbits(bits.view(np.uint8)).sum(dtype=np.int64)) // 2
+    return Graph(bits, n, edges)
+
+
+def objective_bitwise(graph, labels):
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    return lib.cc_objective_bitwise(graph.ptr(), graph.n, labels.ctypes.data_as(_i32p))
+
+
+def objective_direct(graph, labels):
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    return lib.cc_objective_direct(graph.ptr(), graph.n, labels.ctypes.data_as(_i32p))
+
+
+def make_params(k=2, population=128, tournament=5, iterations=100, early_stop=6, perturbation=0.4, gww=0.1, seed=1,
+                time_limit_sec=0.0, verbose=False, ls_kernel="auto", profile=False):
+    p = Params()
+    lib.cc_pbils_default_params(ctypes.byref(p))
+    p.k, p.population, p.tournament, p.iterations, p.early_stop = k, population, tournament, iterations, early_stop
+    p.perturbation, p.gww, p.seed, p.time_limit_sec = perturbation, gww, seed, time_limit_sec
+    p.verbose = int(bool(verbose))
+    p.ls_kernel = LS_KERNELS[ls_kernel] if isinstance(ls_kernel, str) else int(ls_kernel)
+    p.profile = int(bool(profile))
+    return p
+
+
+def _result(stats, labels, n, trace=None):
+    out = stats.as_dict()
+    out["labels"] = labels[:n].copy()
+    if trace is not None:
+        out["trace"] = trace[: stats.iterations_done].copy()
+    return out
+
+
+def _check(rc, err):
+    if rc != OK:
+        raise CcError(rc, err.value.decode(errors="replace"))
+
+
+def solve_pbils(graph, **kw):
+    """SolveGpu: PBILS on the GPU; returns the PbilsResult fields plus the plan, device timings and the trace."""
+    p = make_params(**kw)
+    labels = np.zeros(max(graph.n, 1), dtype=np.int32)
+    trace = np.zeros(max(p.iterations, 1), dtype=np.int64)
+    stats = Stats()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.cc_hip_solve_pbils(graph.ptr(), graph.n, graph.edges, ctypes.byref(p), labels.ctypes.data_as(_i32p),
+                                ctypes.byref(stats), trace.ctypes.data_as(_i64p), err, 512)
+    _check(rc, err)
+    return _result(stats, labels, graph.n, trace)
+
+
+def solve_neighborhood(graph, local_search, **kw):
+    """SolveNeighborhoodGpu (local_search False) / SolveNeighborhoodWithManyLocalSearchesGpu (True)."""
+    p = make_params(**kw)
+    labels = np.zeros(max(graph.n, 1), dtype=np.int32)
+    stats = Stats()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.cc_hip_solve_neighborhood(graph.ptr(), graph.n, graph.edges, ctypes.byref(p), int(bool(local_search)),
+                                       labels.ctypes.data_as(_i32p), ctypes.byref(stats), err, 512)
+    _check(rc, err)
+    return _result(stats, labels, graph.n)
+
+
+class DeviceGraph:
+    """A graph uploaded once to the GPU; solves on it start with the bitset already in HBM."""
+
+    def __init__(self, graph):
+        self.n = graph.n
+        self.edges = graph.edges
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.cc_hip_graph_upload(graph.ptr(), graph.n, graph.edges, ctypes.byref(handle), err, 512)
+        _check(rc, err)
+        self.handle = handle
+
+    def close(self):
+        if self.handle:
+            lib.cc_hip_graph_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def solve_pbils(self, **kw):
+        p = make_params(**kw)
+        labels = np.zeros(max(self.n, 1), dtype=np.int32)
+        trace = np.zeros(max(p.iterations, 1), dtype=np.int64)
+        stats = Stats()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.cc_hip_graph_solve_pbils(self.handle, ctypes.byref(p), labels.ctypes.data_as(_i32p),
+                                          ctypes.byref(stats), trace.ctypes.data_as(_i64p), err, 512)
+        _check(rc, err)
+        return _result(stats, labels, self.n, trace)
+
+    def solve_neighborhood(self, local_search, **kw):
+        p = make_params(**kw)
+        labels = np.zeros(max(self.n, 1), dtype=np.int32)
+        stats = Stats()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.cc_hip_graph_solve_neighborhood(self.handle, ctypes.byref(p), int(bool(local_search)),
+                                                 labels.ctypes.data_as(_i32p), ctypes.byref(stats), err, 512)
+        _check(rc, err)
+        return _result(stats, labels, self.n)
+
+
+def device_info():
+    info = DeviceInfo()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.cc_hip_device_info(ctypes.byref(info), err, 512)
+    _check(rc, err)
+    return {name: (getattr(info, name).decode() if isinstance(getattr(info, name), bytes) else getattr(info, name))
+            for name, _ in info._fields_}

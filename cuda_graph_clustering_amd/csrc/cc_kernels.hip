@@ -1,0 +1,359 @@
+// cc_kernels.hip — population kernels of the PBILS path for gfx950: initial labels, G rebuild, record and GWW
+// ranking, perturbation, Neighborhood start partitions. The descent kernels live in cc_descent.hip.
+//
+// Every kernel of an iteration first reads ctl->stopped and returns when the run is over, so the host can queue
+// (or replay from a HIP graph) whole chunks of iterations without a round trip per iteration and still end on
+// exactly the iteration where the reference's early stop fires.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cc_internal.hpp"
+#include "cc_wave.hpp"
+
+namespace cc::hip {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void KernelResetCtl(Ctl* ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    Ctl c{};
+    c.rec_f = -1;
+    *ctl = c;
+  }
+}
+
+// labels[slot][v] = InitialLabel(init, slot * n + v, k)  (reference: KernelInitLabels, src/cc_pbils_cpu.cpp:198-205)
+__global__ void __launch_bounds__(kBlock) KernelInitLabels(RunDev r) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  const int s = blockIdx.y;
+  if (v >= r.n) return;
+  const uint64_t index = static_cast<uint64_t>(s) * static_cast<uint64_t>(r.n) + static_cast<uint64_t>(v);
+  r.pop.labels[index] = static_cast<uint8_t>(InitialLabel(r.init_stream, index, r.k));
+}
+
+__global__ void __launch_bounds__(kBlock) KernelDegrees(RunDev r) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= r.n) return;
+  const uint64_t* row = r.bits + static_cast<size_t>(v) * r.words;
+  int acc = 0;
+  for (int w = 0; w < r.words; ++w) acc += __popcll(row[w]);
+  r.deg[v] = acc;  // deg is written only here
+}
+
+// Rebuild, part 1: per slot, the cluster masks (one ballot per cluster per 64 labels) and the cluster sizes.
+__global__ void __launch_bounds__(kBlock) KernelMasks(RunDev r) {
+  if (r.ctl->stopped) return;
+  const int s = blockIdx.x;
+  const int n = r.n, k = r.k, words = r.words;
+  const int lane = threadIdx.x & 63;
+  const uint8_t* labels = r.pop.labels + static_cast<size_t>(s) * n;
+  uint64_t* masks = r.masks + static_cast<size_t>(s) * k * words;
+  __shared__ int32_t sizes[kMaxClusters];
+  for (int c = threadIdx.x; c < k; c += kBlock) sizes[c] = 0;
+  __syncthreads();
+  for (int w = WaveIndex(); w < words; w += kBlock / kWave) {
+    const int v = w * kWave + lane;
+    const int label = v < n ? labels[v] : -1;
+    for (int c = 0; c < k; ++c) {
+      const uint64_t m = __ballot(label == c);
+      if (lane == c % kWave) {
+        masks[static_cast<size_t>(c) * words + w] = m;
+        if (m) atomicAdd(&sizes[c], __popcll(m));
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += kBlock) r.pop.sizes[static_cast<size_t>(s) * k + c] = sizes[c];
+  if (threadIdx.x == 0) r.intra[s] = 0;
+}
+
+// Rebuild, part 2: G[s][c][v] = sum_w popc(row_v[w] & mask_{s,c}[w]) for a tile of kRows vertices x kSlots slots x
+// kClusters clusters. The masks of the tile are staged through LDS in chunks of kWordTile words; every thread owns
+// one row. The diagonal G[s][label v][v] is summed into intra[s] for the objective.
+constexpr int kRebuildRows = 256;
+constexpr int kRebuildSlots = 2;
+constexpr int kWordTile = 256;
+
+template <typename GT, int kClusters>
+__global__ void __launch_bounds__(kRebuildRows) KernelRebuildG(RunDev r) {
+  if (r.ctl->stopped) return;
+  const int n = r.n, k = r.k, words = r.words, population = r.population;
+  const int v = blockIdx.x * kRebuildRows + threadIdx.x;
+  const int s0 = blockIdx.y * kRebuildSlots;
+  const int c0 = blockIdx.z * kClusters;
+  __shared__ uint64_t tile[kRebuildSlots][kClusters][kWordTile];
+  __shared__ int64_t block_intra[kRebuildSlots];
+  if (threadIdx.x < kRebuildSlots) block_intra[threadIdx.x] = 0;
+
+  int acc[kRebuildSlots][kClusters];
+#pragma unroll
+  for (int i = 0; i < kRebuildSlots; ++i)
+#pragma unroll
+    for (int c = 0; c < kClusters; ++c) acc[i][c] = 0;
+
+  const uint64_t* row = r.bits + static_cast<size_t>(v < n ? v : 0) * words;
+  for (int w0 = 0; w0 < words; w0 += kWordTile) {
+    const int span = words - w0 < kWordTile ? words - w0 : kWordTile;
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRebuildSlots * kClusters * kWordTile; i += kRebuildRows) {
+      const int w = i % kWordTile;
+      const int c = (i / kWordTile) % kClusters;
+      const int si = i / (kWordTile * kClusters);
+      const int s = s0 + si;
+      uint64_t m = 0;
+      if (w < span && s < population && c0 + c < k) {
+        m = r.masks[(static_cast<size_t>(s) * k + c0 + c) * words + w0 + w];
+      }
+      tile[si][c][w] = m;
+    }
+    __syncthreads();
+    if (v < n) {
+      for (int w = 0; w < span; ++w) {
+        const uint64_t x = row[w0 + w];
+#pragma unroll
+        for (int i = 0; i < kRebuildSlots; ++i)
+#pragma unroll
+          for (int c = 0; c < kClusters; ++c) acc[i][c] += __popcll(x & tile[i][c][w]);
+      }
+    }
+  }
+  if (v < n) {
+#pragma unroll
+    for (int i = 0; i < kRebuildSlots; ++i) {
+      const int s = s0 + i;
+      if (s >= population) continue;
+      GT* g = static_cast<GT*>(r.pop.g) + static_cast<size_t>(s) * k * n;
+      const int label = r.pop.labels[static_cast<size_t>(s) * n + v];
+      int diag = 0;
+#pragma unroll
+      for (int c = 0; c < kClusters; ++c) {
+        if (c0 + c < k) g[static_cast<size_t>(c0 + c) * n + v] = static_cast<GT>(acc[i][c]);
+        if (label == c0 + c) diag = acc[i][c];
+      }
+      if (diag) atomicAdd(reinterpret_cast<unsigned long long*>(&block_intra[i]), static_cast<unsigned long long>(diag));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kRebuildSlots && s0 + static_cast<int>(threadIdx.x) < population && block_intra[threadIdx.x]) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&r.intra[s0 + threadIdx.x]),
+              static_cast<unsigned long long>(block_intra[threadIdx.x]));
+  }
+}
+
+// Rebuild, part 3: f = m + sum_c C(s_c, 2) - 2W with 2W = intra (reference include/cc_math.hpp:66-68).
+__global__ void __launch_bounds__(kBlock) KernelFinalize(RunDev r) {
+  if (r.ctl->stopped) return;
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= r.population) return;
+  const int32_t* sizes = r.pop.sizes + static_cast<size_t>(s) * r.k;
+  r.pop.f[s] = r.edges + SamePairs(sizes, r.k) - r.intra[s];
+}
+
+// Best (f, slot) of a block: the first minimum.
+struct FSlot {
+  int64_t f;
+  int32_t slot;
+};
+
+__device__ __forceinline__ bool Better(const FSlot& a, const FSlot& b) {
+  return a.f < b.f || (a.f == b.f && a.slot < b.slot);
+}
+
+__device__ FSlot BlockArgMin(FSlot x, FSlot* scratch /*[16]*/) {
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    FSlot o;
+    o.f = __shfl_xor(x.f, offset, 64);
+    o.slot = __shfl_xor(x.slot, offset, 64);
+    if (Better(o, x)) x = o;
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) scratch[wave] = x;
+  __syncthreads();
+  FSlot best = scratch[0];
+  const int waves = (blockDim.x + 63) >> 6;
+  for (int w = 1; w < waves; ++w) {
+    if (Better(scratch[w], best)) best = scratch[w];
+  }
+  return best;
+}
+
+__device__ int64_t BlockSum(int64_t x, int64_t* scratch /*[16]*/) {
+  for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[wave] = x;
+  __syncthreads();
+  int64_t total = 0;
+  const int waves = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < waves; ++w) total += scratch[w];
+  return total;
+}
+
+constexpr int kRecordThreads = 1024;
+constexpr int kRankLdsSlots = 4096;
+
+// The record after a phase: the first slot of minimal f, taken only if strictly better (reference
+// src/cc_pbils_cpu.cpp:208-214, cuda/cc_pbils_gpu.cu:324-334). In iteration mode it also does Progress::Next
+// (include/cc_pbils.hpp:69-78) and the GWW ranking by (f, slot) (src/cc_pbils_cpu.cpp:168-175, 234-239), leaving in
+// src[] the slot of `next` that each new slot copies.
+__global__ void __launch_bounds__(kRecordThreads) KernelRecord(RunDev r, int mode, int read_next, int32_t* order) {
+  Ctl* ctl = r.ctl;
+  if (ctl->stopped) return;
+  const PopDev& p = read_next ? r.next : r.pop;
+  const int population = r.population;
+  const int tid = threadIdx.x;
+  __shared__ FSlot scratch[16];
+  __shared__ int64_t sums[16];
+  __shared__ int32_t improved_sh, stop_sh;
+  __shared__ int64_t f_lds[kRankLdsSlots];
+
+  FSlot mine{INT64_MAX, INT32_MAX};
+  for (int s = tid; s < population; s += kRecordThreads) {
+    const FSlot c{p.f[s], s};
+    if (Better(c, mine)) mine = c;
+  }
+  const FSlot best = BlockArgMin(mine, scratch);
+  if (tid == 0) {
+    const bool improved = !(ctl->have_record && r.n > 0 && best.f >= ctl->rec_f);
+    if (improved) {
+      ctl->rec_f = best.f;
+      ctl->rec_slot = best.slot;
+      ctl->have_record = 1;
+    }
+    improved_sh = improved;
+  }
+  __syncthreads();
+  const bool improved = improved_sh;
+  if (improved) {
+    const uint8_t* from = p.labels + static_cast<size_t>(best.slot) * r.n;
+    for (int v = tid; v < r.n; v += kRecordThreads) r.rec_labels[v] = from[v];
+  }
+  if (mode == kRecordInitial) return;
+
+  int64_t moved = 0;
+  for (int s = tid; s < population; s += kRecordThreads) moved += r.moves[s];
+  moved = BlockSum(moved, sums);
+  if (tid == 0) ctl->accepted_moves += moved;
+  if (mode == kRecordBatch) return;
+
+  if (tid == 0) {
+    ctl->iterations_done += 1;
+    ctl->local_searches += population;
+    ctl->stall = improved ? 0 : ctl->stall + 1;
+    r.trace[ctl->iterations_done - 1] = ctl->rec_f;
+    const bool stop = ctl->stall >= r.early_stop || ctl->iterations_done >= r.iterations;
+    stop_sh = stop;
+    if (stop) ctl->stopped = 1;
+  }
+  __syncthreads();
+  if (stop_sh) return;
+
+  for (int s = tid; s < population; s += kRecordThreads) r.src[s] = s;
+  if (r.copies > 0) {
+    const bool staged = population <= kRankLdsSlots;
+    if (staged) {
+      for (int s = tid; s < population; s += kRecordThreads) f_lds[s] = p.f[s];
+    }
+    __syncthreads();
+    const int64_t* f = staged ? f_lds : p.f;
+    // rank(s) = #{t : (f_t, t) < (f_s, s)}: a stable sort by f, ties by slot.
+    for (int s = tid; s < population; s += kRecordThreads) {
+      const int64_t fs = f[s];
+      int rank = 0;
+      for (int t = 0; t < population; ++t) {
+        const int64_t ft = f[t];
+        rank += (ft < fs) | ((ft == fs) & (t < s));
+      }
+      order[rank] = s;
+    }
+    __syncthreads();
+    for (int i = tid; i < r.copies; i += kRecordThreads) r.src[order[population - 1 - i]] = order[i];
+  }
+  if (tid == 0) {
+    ctl->perturb_iteration = ctl->iteration;
+    ctl->iteration += 1;
+  }
+}
+
+// The next population: slot s starts from next[src[s]] (GWW) and is perturbed with the per-label stream
+// (reference KernelPerturb and Perturb, src/cc_pbils_cpu.cpp:151-161: index = slot * n + v).
+__global__ void __launch_bounds__(kBlock) KernelPerturb(RunDev r) {
+  if (r.ctl->stopped) return;
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  const int s = blockIdx.y;
+  if (v >= r.n) return;
+  const int from = r.src[s];
+  int label = r.next.labels[static_cast<size_t>(from) * r.n + v];
+  if (r.do_perturb) {
+    const uint64_t index = static_cast<uint64_t>(s) * static_cast<uint64_t>(r.n) + static_cast<uint64_t>(v);
+    label = PerturbedLabel(r.perturb_stream, index, r.ctl->perturb_iteration, label, r.k, r.perturbation);
+  }
+  r.pop.labels[static_cast<size_t>(s) * r.n + v] = static_cast<uint8_t>(label);
+}
+
+// Start partition of start vertex first + b: v and its neighbours -> 0, the rest -> 1 (reference
+// KernelNeighborhoodLabels, cuda/cc_population_kernels.cuh:15-22).
+__global__ void __launch_bounds__(kBlock) KernelNeighborhoodLabels(RunDev r, int first) {
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  const int b = blockIdx.y;
+  if (u >= r.n) return;
+  const int v = first + b;
+  const uint64_t* row = r.bits + static_cast<size_t>(v) * r.words;
+  const bool joined = (row[u >> 6] >> (u & 63)) & 1ull;
+  r.pop.labels[static_cast<size_t>(b) * r.n + u] = (u == v || joined) ? 0 : 1;
+}
+
+}  // namespace
+
+void LaunchResetCtl(const RunDev& r, hipStream_t stream) { KernelResetCtl<<<1, 64, 0, stream>>>(r.ctl); }
+
+void LaunchInitLabels(const RunDev& r, hipStream_t stream) {
+  const dim3 grid((r.n + kBlock - 1) / kBlock, r.population);
+  KernelInitLabels<<<grid, kBlock, 0, stream>>>(r);
+}
+
+void LaunchDegrees(const RunDev& r, hipStream_t stream) {
+  KernelDegrees<<<(r.n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(r);
+}
+
+template <typename GT>
+static void LaunchRebuildG(const RunDev& r, hipStream_t stream) {
+  const int row_blocks = (r.n + kRebuildRows - 1) / kRebuildRows;
+  const int slot_blocks = (r.population + kRebuildSlots - 1) / kRebuildSlots;
+  if (r.k <= 2) {
+    KernelRebuildG<GT, 2><<<dim3(row_blocks, slot_blocks, 1), kRebuildRows, 0, stream>>>(r);
+  } else if (r.k <= 4) {
+    KernelRebuildG<GT, 4><<<dim3(row_blocks, slot_blocks, 1), kRebuildRows, 0, stream>>>(r);
+  } else {
+    KernelRebuildG<GT, 8><<<dim3(row_blocks, slot_blocks, (r.k + 7) / 8), kRebuildRows, 0, stream>>>(r);
+  }
+}
+
+void LaunchRebuild(const RunDev& r, int wide, hipStream_t stream) {
+  KernelMasks<<<r.population, kBlock, 0, stream>>>(r);
+  if (wide) {
+    LaunchRebuildG<int32_t>(r, stream);
+  } else {
+    LaunchRebuildG<int16_t>(r, stream);
+  }
+  KernelFinalize<<<(r.population + kBlock - 1) / kBlock, kBlock, 0, stream>>>(r);
+}
+
+// `order` is a [population] int32 scratch for the GWW ranking.
+void LaunchRecord(const RunDev& r, int mode, int read_next, int32_t* order, hipStream_t stream) {
+  KernelRecord<<<1, kRecordThreads, 0, stream>>>(r, mode, read_next, order);
+}
+
+void LaunchPerturb(const RunDev& r, hipStream_t stream) {
+  const dim3 grid((r.n + kBlock - 1) / kBlock, r.population);
+  KernelPerturb<<<grid, kBlock, 0, stream>>>(r);
+}
+
+void LaunchNeighborhoodLabels(const RunDev& r, int first, int count, hipStream_t stream) {
+  const dim3 grid((r.n + kBlock - 1) / kBlock, count);
+  KernelNeighborhoodLabels<<<grid, kBlock, 0, stream>>>(r, first);
+}
+
+}  // namespace cc::hip

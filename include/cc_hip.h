@@ -1,0 +1,148 @@
+/*
+ * cc_hip.h — the C ABI of the MI355X PBILS correlation-clustering library (libcc_hip.so).
+ *
+ * Plain C: POD structs, raw pointers and sizes, integer status codes. The C++ entry points of the reference
+ * (include/cc_gpu.hpp:12-20 of the reference) are re-declared in cc_gpu.hpp over this ABI; any other host language
+ * binds these functions directly (see INTEGRATION.md for a ctypes and a JNI-style binding).
+ *
+ * Graph layout everywhere: n rows of words64(n) = max(1, ceil(n/64)) uint64 words, row-major; bit j of row i is
+ * bit (j % 64) of word j / 64. The matrix is symmetric with a zero diagonal and zero padding bits.
+ *
+ * Status codes: CC_HIP_OK (0) or one of the CC_HIP_E* codes; a message is written to err (if err_len > 0). The
+ * reference throws std::runtime_error for the same conditions; cc_gpu.hpp rethrows them as such.
+ */
+#ifndef CC_HIP_H
+#define CC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CC_HIP_OK 0
+#define CC_HIP_EINVAL 1  /* invalid parameters or graph (reference: std::runtime_error from CheckParams etc.) */
+#define CC_HIP_ENOMEM 2  /* device memory pre-flight failed (reference: cuda/cc_pbils_gpu.cu:198-207) */
+#define CC_HIP_EDEVICE 3 /* HIP runtime error (reference: CC_CUDA_CHECK, cuda/cc_gpu_common.cuh:13-20) */
+#define CC_HIP_ENODEV 4  /* no usable device */
+
+/* Largest k the kernels hold (reference include/cc_gpu.hpp:10). */
+#define CC_HIP_MAX_CLUSTERS 64
+/* Largest n of the 16-bit-G narrow path and of any path (reference cuda/cc_gpu_common.cuh:29-32). */
+#define CC_HIP_MAX_VERTICES_NARROW 32767
+#define CC_HIP_MAX_VERTICES_WIDE (1 << 20)
+
+/* PbilsParams::LocalSearchKernel (reference include/cc_pbils.hpp:30-35). */
+#define CC_LS_AUTO 0
+#define CC_LS_GLOBAL 1
+#define CC_LS_SHARED 2
+#define CC_LS_WIDE 3
+
+/* Mirrors PbilsParams (reference include/cc_pbils.hpp:26-49); same defaults via cc_pbils_default_params. */
+typedef struct cc_pbils_params {
+  int32_t k;
+  int32_t population;
+  int32_t tournament;
+  int32_t iterations;
+  int32_t early_stop;
+  int32_t threads; /* CPU-only in the reference; ignored here */
+  double perturbation;
+  double gww;
+  uint64_t seed;
+  double time_limit_sec; /* 0 disables; checked between iteration chunks */
+  int32_t verbose;
+  int32_t ls_kernel; /* CC_LS_* */
+  int32_t profile;   /* 1: time the search and rebuild kernels with HIP events (fills the *_ms fields) */
+  int32_t reserved;
+} cc_pbils_params;
+
+/* Mirrors PbilsResult (reference include/cc_pbils.hpp:51-59) plus the plan and device timings. */
+typedef struct cc_pbils_stats {
+  int64_t objective;
+  int32_t iterations_done;
+  int32_t clusters_used;
+  int64_t local_searches;
+  int64_t accepted_moves;
+  double seconds; /* solver clock: from entry (allocation and upload included, as the reference) to exit */
+  /* plan actually used */
+  int32_t descent;      /* CC_DESCENT_* */
+  int32_t threads;      /* threads per block of the descent kernel */
+  int32_t blocks;       /* blocks of the descent kernel */
+  int32_t vertices_per_thread;
+  int64_t lds_bytes;    /* dynamic LDS of the descent kernel */
+  /* device timings, filled when params.profile != 0 */
+  double search_ms;     /* sum over descent launches */
+  double rebuild_ms;    /* sum over G rebuilds */
+  double iteration_ms;  /* device time of all PBILS iterations */
+  int64_t search_launches;
+  int64_t rebuild_launches;
+} cc_pbils_stats;
+
+#define CC_DESCENT_GENERIC_GLOBAL 0
+#define CC_DESCENT_GENERIC_LDS 1
+#define CC_DESCENT_K2_REGISTERS 2
+#define CC_DESCENT_K2_WIDE 3
+#define CC_DESCENT_K3_WIDE 4
+
+void cc_pbils_default_params(cc_pbils_params* params);
+
+/* ---- one-shot solves on a host-resident graph ------------------------------------------------------------ */
+
+/* Replaces cc::SolveGpu (reference include/cc_gpu.hpp:12; cuda/cc_pbils_gpu.cu:441-454). labels_out: n entries.
+ * trace_out (nullable): params.iterations entries, the record after each completed iteration. */
+int cc_hip_solve_pbils(const uint64_t* bits, uint32_t n, uint64_t edges, const cc_pbils_params* params,
+                       int32_t* labels_out, cc_pbils_stats* stats, int64_t* trace_out, char* err, size_t err_len);
+
+/* Replaces cc::SolveNeighborhoodGpu (local_search = 0) and cc::SolveNeighborhoodWithManyLocalSearchesGpu
+ * (local_search = 1) (reference include/cc_gpu.hpp:19-20; cuda/cc_pbils_gpu.cu:389-436). params.population is the
+ * batch of start partitions, params.ls_kernel picks the descent; k must be 2. */
+int cc_hip_solve_neighborhood(const uint64_t* bits, uint32_t n, uint64_t edges, const cc_pbils_params* params,
+                              int32_t local_search, int32_t* labels_out, cc_pbils_stats* stats, char* err,
+                              size_t err_len);
+
+/* ---- a device-resident graph: upload once, solve many times (no reference counterpart: the reference uploads
+ *      the bits on every call, cuda/cc_pbils_gpu.cu:212) ----------------------------------------------------- */
+
+typedef struct cc_hip_graph cc_hip_graph;
+
+int cc_hip_graph_upload(const uint64_t* bits, uint32_t n, uint64_t edges, cc_hip_graph** out, char* err,
+                        size_t err_len);
+void cc_hip_graph_free(cc_hip_graph* graph);
+int cc_hip_graph_solve_pbils(cc_hip_graph* graph, const cc_pbils_params* params, int32_t* labels_out,
+                             cc_pbils_stats* stats, int64_t* trace_out, char* err, size_t err_len);
+int cc_hip_graph_solve_neighborhood(cc_hip_graph* graph, const cc_pbils_params* params, int32_t local_search,
+                                    int32_t* labels_out, cc_pbils_stats* stats, char* err, size_t err_len);
+
+/* ---- host-side graph generators and recounts of the product (the reference's Graph::ErdosRenyi,
+ *      src/cc_graph.cpp:168-177, and the recounts of src/cc_pbils_cpu.cpp:12-53) --------------------------- */
+
+uint32_t cc_words64(uint32_t n);
+/* Each returns the edge count and fills bits (n * cc_words64(n) words). threads = 0: all host cores. */
+uint64_t cc_graph_erdos_renyi(uint32_t n, double p, uint64_t seed, uint64_t* bits, int32_t threads);
+uint64_t cc_graph_sbm(uint32_t n, int32_t blocks, double p_in, double p_out, uint64_t seed, uint64_t* bits,
+                      int32_t threads);
+uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_t seed, uint64_t* bits,
+                           int32_t threads);
+int64_t cc_objective_direct(const uint64_t* bits, uint32_t n, const int32_t* labels);
+int64_t cc_objective_bitwise(const uint64_t* bits, uint32_t n, const int32_t* labels);
+
+/* ---- device information ---------------------------------------------------------------------------------- */
+
+typedef struct cc_device_info {
+  char name[64];
+  char arch[32];
+  int32_t compute_units;
+  int32_t lds_per_block;
+  int32_t l2_bytes;
+  int32_t clock_khz;
+  int64_t total_bytes;
+  int64_t free_bytes;
+} cc_device_info;
+
+int cc_hip_device_info(cc_device_info* info, char* err, size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""PBILS on the GPU against the oracle: the same graph and --seed must give the same trajectory — the same record
+after every iteration, the same final labels, objective, iterations done and accepted-move count — for every
+descent kernel (reference claim README.md:62-64; the check the reference only made in docs/plot_benchmarks.py:145).
+All results are integers and compared exactly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("objective", "iterations_done", "clusters_used", "local_searches", "accepted_moves")
+
+
+@pytest.fixture(scope="module")
+def cga():
+    import cuda_graph_clustering_amd as m
+
+    return m
+
+
+def as_oracle_graph(g):
+    import oracle
+
+    return oracle.Graph(g.bits, g.n, g.edges)
+
+
+def check_same(got, want, context):
+    for key in KEYS:
+        assert got[key] == want[key], f"{context}: {key} {got[key]} != {want[key]}"
+    assert (got["labels"] == want["labels"]).all(), f"{context}: labels"
+    if "trace" in want:
+        assert (got["trace"] == want["trace"]).all(), f"{context}: record trace"
+
+
+def run_both(cga, orc, g, cpu_threads, **kw):
+    got = cga.solve_pbils(g, **kw)
+    params = {k: v for k, v in kw.items() if k != "ls_kernel"}
+    want = orc.solve_pbils(as_oracle_graph(g), threads=cpu_threads, trace=True, **params)
+    return got, want
+
+
+CASES = [
+    # n, generator, k, pop, iterations, early_stop, perturbation, gww, seed
+    (1, "er", 2, 4, 3, 3, 0.4, 0.1, 1),
+    (2, "er", 2, 3, 4, 4, 0.4, 0.0, 2),
+    (31, "er", 2, 16, 8, 8, 0.4, 0.1, 3),
+    (64, "er", 3, 16, 8, 8, 0.4, 0.25, 4),
+    (65, "sbm", 5, 12, 10, 10, 0.3, 0.1, 5),
+    (129, "er", 1, 8, 5, 5, 0.4, 0.1, 6),
+    (200, "sbm", 8, 20, 6, 6, 0.0, 0.1, 7),
+    (150, "cl", 4, 9, 6, 2, 0.5, 0.5, 8),
+    (97, "er", 64, 5, 4, 4, 0.4, 0.2, 9),
+    (300, "sbm", 2, 33, 12, 3, 0.4, 0.1, 10),
+    (257, "er", 3, 1, 6, 6, 0.4, 0.0, 11),
+]
+
+
+def make_graph(cga, kind, n, seed):
+    if kind == "er":
+        return cga.erdos_renyi(n, 0.4, seed)
+    if kind == "sbm":
+        return cga.sbm(n, 4, 0.4, 0.05, seed)
+    return cga.chung_lu(n, 2.5, 8.0, seed)
+
+
+@pytest.mark.parametrize("ls_kernel", ["auto", "global", "shared", "wide"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c[0]}-{c[1]}-k{c[2]}-p{c[3]}")
+def test_trajectory_matches_oracle(cga, orc, cpu_threads, case, ls_kernel):
+    n, kind, k, pop, iters, es, pert, gww, seed = case
+    g = make_graph(cga, kind, n, seed)
+    got, want = run_both(cga, orc, g, cpu_threads, k=k, population=pop, iterations=iters, early_stop=es,
+                         perturbation=pert, gww=gww, seed=seed, ls_kernel=ls_kernel)
+    check_same(got, want, f"n={n} k={k} kernel={ls_kernel}")
+    assert got["objective"] == cga.objective_direct(g, got["labels"])
+
+
+def test_c1_full_config(cga, orc, cpu_threads):
+    """ER G(1000, 0.5) graph-seed 1, k = 2, pop 128, 100 iterations without early stop, gww 0.1, seed 42."""
+    g = cga.erdos_renyi(1000, 0.5, 1)
+    got, want = run_both(cga, orc, g, cpu_threads, k=2, population=128, iterations=100, early_stop=100, gww=0.1,
+                         seed=42)
+    check_same(got, want, "C1")
+    assert got["descent_name"] == "k2-registers"
+
+
+@pytest.mark.parametrize("ls_kernel", ["auto", "global", "shared"])
+def test_golden_gpu_csv_rows(cga, cpu_threads, ls_kernel):
+    """docs/data/gpu.csv of the reference: n=1000, k=3, pop 128, 20 iterations, gww 0, graph-seed 7, seeds 6-8."""
+    import csv
+    import os
+
+    rows = [r for r in csv.DictReader(open(os.path.join(os.path.dirname(__file__), "golden", "gpu_trajectories.csv")))
+            if r["series"] == "n128" and r["n"] == "1000"]
+    g = cga.erdos_renyi(1000, 0.33, 7)
+    for r in rows:
+        got = cga.solve_pbils(g, k=3, population=128, iterations=20, early_stop=1000, gww=0.0, seed=int(r["seed"]),
+                              ls_kernel=ls_kernel)
+        assert got["objective"] == int(r["f"])
+
+
+def test_golden_gww_curves(cga):
+    """docs/data/gww.csv of the reference: the record after iterations 1..50 at n=1000, k=5, pop 128."""
+    import csv
+    import os
+
+    rows = list(csv.DictReader(open(os.path.join(os.path.dirname(__file__), "golden", "gww_records.csv"))))
+    curves = {}
+    for r in rows:
+        if r["n"] != "1000":
+            continue
+        curves.setdefault((int(r["graph_seed"]), float(r["gww"]), int(r["seed"])), {})[int(r["iter"])] = int(r["record"])
+    graphs = {s: cga.erdos_renyi(1000, 0.33, s) for s in (7, 8)}
+    checked = 0
+    for (gs, gww, seed), points in sorted(curves.items()):
+        if seed not in (2, 3):
+            continue
+        last = max(points)
+        got = cga.solve_pbils(graphs[gs], k=5, population=128, iterations=last, early_stop=last, gww=gww, seed=seed)
+        for it, record in points.items():
+            assert got["trace"][it - 1] == record, (gs, gww, seed, it)
+            checked += 1
+    assert checked > 50
+
+
+def test_golden_early_stop(cga):
+    """docs/data/gww_stop.csv of the reference: default settings with early stop 6: final f and iterations done."""
+    import csv
+    import os
+
+    rows = [r for r in csv.DictReader(open(os.path.join(os.path.dirname(__file__), "golden", "gww_stop.csv")))
+            if r["n"] in ("1000", "2000") and int(r["seed"]) <= 4]
+    graphs = {}
+    for r in rows:
+        key = (int(r["n"]), int(r["graph_seed"]))
+        if key not in graphs:
+            graphs[key] = cga.erdos_renyi(key[0], 0.33, key[1])
+        got = cga.solve_pbils(graphs[key], k=int(r["k"]), population=int(r["pop"]), gww=float(r["gww"]),
+                              seed=int(r["seed"]))
+        assert (got["objective"], got["iterations_done"]) == (int(r["f"]), int(r["iters"])), r
+
+
+def test_invalid_input_rejected(cga):
+    g = cga.erdos_renyi(10, 0.5, 1)
+    for kw in ({"k": 0}, {"population": 0}, {"gww": 0.6}, {"gww": -0.1}, {"k": 65}):
+        with pytest.raises(cga.CcError) as e:
+            cga.solve_pbils(g, **kw)
+        assert e.value.code == cga.EINVAL
+    with pytest.raises(cga.CcError):
+        cga.solve_pbils(cga.erdos_renyi(0, 0.5, 1))
+    # the library stays usable after rejected calls
+    assert cga.solve_pbils(g, k=3, iterations=1)["iterations_done"] == 1
+
+
+def test_shared_kernel_rejects_state_larger_than_lds(cga):
+    g = cga.erdos_renyi(3000, 0.1, 1)
+    with pytest.raises(cga.CcError) as e:
+        cga.solve_pbils(g, k=64, population=2, iterations=1, ls_kernel="shared")
+    assert e.value.code == cga.EINVAL
+
+
+def test_device_graph_repeated_solves_are_identical(cga):
+    g = cga.sbm(500, 4, 0.4, 0.05, 3)
+    with cga.DeviceGraph(g) as dg:
+        a = dg.solve_pbils(k=4, population=32, iterations=10, early_stop=10, seed=5)
+        b = dg.solve_pbils(k=4, population=32, iterations=10, early_stop=10, seed=5)
+    c = cga.solve_pbils(g, k=4, population=32, iterations=10, early_stop=10, seed=5)
+    for key in KEYS:
+        assert a[key] == b[key] == c[key]
+    assert (a["labels"] == b["labels"]).all() and (a["labels"] == c["labels"]).all()
