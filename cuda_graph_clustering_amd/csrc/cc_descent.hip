@@ -285,6 +285,185 @@ __global__ void __launch_bounds__(T) KernelDescentK2(RunDev r) {
   }
 }
 
+// Adjacency rows are read-only for the whole run: reading them through the constant address space lets uniform
+// row words come in with scalar loads.
+using ConstWords = const __attribute__((address_space(4))) uint64_t*;
+
+// k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt. Per vertex:
+//   gs[c] = 128 G[v][c]   and   keya = key of its own cluster,
+// where the key of target c is  key_c = ((2 G_c - s_c + kOff) << 6) | (63 - c) = gs[c] + K_c  with the wave-uniform
+// K_c = ((kOff - s_c) << 6) + 63 - c. The largest key gives the largest 2 G_c - s_c, ties to the smaller c, and
+//   delta(a -> c) = (s_c - s_a + 1) - 2 (G_c - G_a) = (key_a >> 6) - (key_c >> 6) + 1.
+// Including the own cluster a in the maximum is harmless: a move is improving only if its key is strictly above
+// key_a's q by more than one, and then a is not the maximum. A move of v: from -> to changes, for each neighbour u,
+// gs[from] -= 128 and gs[to] += 128; for every vertex with label a it shifts keya by t_a (128 bit - 64) with
+// t_a = [a == to] - [a == from] (G change of a neighbour plus the change of K_a through s_a); t_a is fetched per
+// lane with ds_bpermute from a lane table.
+template <int T, int K, int kVpt>
+__global__ void __launch_bounds__(T) KernelDescentRegs(RunDev r) {
+  Ctl* ctl = r.ctl;
+  if (ctl->stopped) return;
+  constexpr int kWaves = T / kWave;
+  constexpr int kOff = 1 << 16;
+  constexpr int kIdleKey = 1 << 30;  // own key of a vertex beyond n: never improvable
+  static_assert(K >= 3 && K <= 8, "register descent handles k in [3, 8]");
+  const int s = blockIdx.x;
+  const int n = r.n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = WaveIndex();
+  __shared__ uint64_t partial[2][kWaves];
+  __shared__ int32_t src_sh;
+  if (tid == 0) {
+    src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
+  }
+  __syncthreads();
+  const int src = __builtin_amdgcn_readfirstlane(src_sh);
+  const int16_t* g_in = static_cast<const int16_t*>(r.pop.g) + static_cast<size_t>(src) * K * n;
+  const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
+  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * K;
+
+  int sz[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) sz[c] = __builtin_amdgcn_readfirstlane(sizes_in[c]);
+  auto kconst = [&](int c) { return ((kOff - sz[c]) << 6) + (63 - c); };
+
+  int gs[kVpt][K];
+  int lab[kVpt];
+  int keya[kVpt];
+#pragma unroll
+  for (int j = 0; j < kVpt; ++j) {
+    const int v = tid + j * T;
+    lab[j] = 0;
+    keya[j] = kIdleKey;
+#pragma unroll
+    for (int c = 0; c < K; ++c) gs[j][c] = 0;
+    if (v < n) {
+      lab[j] = lab_in[v];
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        gs[j][c] = static_cast<int>(g_in[static_cast<size_t>(c) * n + v]) << 7;
+        if (lab[j] == c) keya[j] = gs[j][c] + kconst(c);
+      }
+    }
+  }
+  const int64_t f_start = r.pop.f[src];
+  int64_t objective = f_start;
+  int64_t applied = 0;
+  const int64_t cap = MoveCap(f_start, n);
+
+  int kc[K];
+  auto scan = [&]() {
+#pragma unroll
+    for (int c = 0; c < K; ++c) kc[c] = kconst(c);
+    int best_d = 0, best_j = -1, best_key = 0, best_a = 0;
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      int kb = gs[j][0] + kc[0];
+#pragma unroll
+      for (int c = 1; c < K; ++c) kb = max(kb, gs[j][c] + kc[c]);
+      const int d = (keya[j] >> 6) - (kb >> 6) + 1;
+      if (d < best_d) {
+        best_d = d;
+        best_j = j;
+        best_key = kb;
+        best_a = lab[j];
+      }
+    }
+    if (best_j < 0) return kNoMove;
+    return PackMove(best_d, tid + best_j * T, best_a, 63 - (best_key & 63));
+  };
+
+  uint64_t key = scan();
+  int parity = 0;
+  const ConstWords bits = (ConstWords)r.bits;  // address-space cast (global -> constant)
+  while (true) {
+    const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
+    parity ^= 1;
+    if (best == kNoMove) break;
+    const int delta = static_cast<int>(static_cast<uint32_t>(best >> 32)) - kDeltaOffset;
+    const uint32_t mv = static_cast<uint32_t>(best);
+    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(mv >> 12));
+    const int from = __builtin_amdgcn_readfirstlane(static_cast<int>((mv >> 6) & 63u));
+    const int to = __builtin_amdgcn_readfirstlane(static_cast<int>(mv & 63u));
+    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
+    uint64_t words[kVpt];
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const int base = wave * kWave + j * T;  // wave-uniform
+      words[j] = base < n ? row[base >> 6] : 0ull;
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) sz[c] += (c == to) - (c == from);
+    // lane c of the table holds t_c = [c == to] - [c == from]
+    const int table = (lane == to) - (lane == from);
+    int b7[kVpt];
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const int bit = static_cast<int>((words[j] >> lane) & 1ull);
+      b7[j] = bit << 7;
+      const int ta = __builtin_amdgcn_ds_bpermute(lab[j] << 2, table);
+      keya[j] += ta * (b7[j] - 64);
+    }
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      if (c == from) {
+#pragma unroll
+        for (int j = 0; j < kVpt; ++j) gs[j][c] -= b7[j];
+      }
+      if (c == to) {
+#pragma unroll
+        for (int j = 0; j < kVpt; ++j) gs[j][c] += b7[j];
+      }
+    }
+    // the moved vertex: new label, own key from its (unchanged) G and the new sizes
+    const int mv_j = v / T;
+    if (tid == v - mv_j * T) {
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) {
+        if (j == mv_j) {
+          lab[j] = to;
+#pragma unroll
+          for (int c = 0; c < K; ++c) {
+            if (c == to) keya[j] = gs[j][c] + kconst(c);
+          }
+        }
+      }
+    }
+    key = scan();
+    objective += delta;
+    if (++applied >= cap) {
+      if (tid == 0) {
+        ctl->error = 1;
+        ctl->stopped = 1;
+      }
+      break;
+    }
+  }
+
+  uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
+  int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * K * n;
+#pragma unroll
+  for (int j = 0; j < kVpt; ++j) {
+    const int v = tid + j * T;
+    if (v < n) {
+      lab_out[v] = static_cast<uint8_t>(lab[j]);
+#pragma unroll
+      for (int c = 0; c < K; ++c) g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>(gs[j][c] >> 7);
+    }
+  }
+  if (tid < K) {
+    int size = 0;
+#pragma unroll
+    for (int c = 0; c < K; ++c) size = tid == c ? sz[c] : size;
+    r.next.sizes[static_cast<size_t>(s) * K + tid] = size;
+  }
+  if (tid == 0) {
+    r.next.f[s] = objective;
+    r.moves[s] = applied;
+  }
+}
+
 constexpr size_t kLdsBytes = 160 * 1024;
 // Static LDS of the descent kernels (partials, sizes, source slot), rounded up.
 constexpr size_t kStaticLdsReserve = 2048;
@@ -342,6 +521,56 @@ bool LaunchK2Vpt(const RunDev& r, const DescentPlan& plan, hipStream_t stream, b
   }
 }
 
+// Register descent instances: T = 256 with kVpt in {1, 2, 4}; T = 1024 with kVpt in {2, 4, 6, 8, 10, 12},
+// bounded by kVpt (K + 5) <= 112 so that the state fits the 128 VGPRs of a 1024-thread workgroup.
+constexpr bool RegsFit(int k, int vpt) { return vpt * (k + 5) <= 112; }
+
+template <int T, int K, int kVpt>
+bool LaunchRegsOne(const RunDev& r, hipStream_t stream, bool launch) {
+  if constexpr (RegsFit(K, kVpt)) {
+    if (launch) KernelDescentRegs<T, K, kVpt><<<r.population, T, 0, stream>>>(r);
+    return true;
+  } else {
+    return false;
+  }
+}
+
+template <int K>
+bool LaunchRegsK(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
+  if (plan.threads == 256) {
+    switch (plan.vpt) {
+      case 1: return LaunchRegsOne<256, K, 1>(r, stream, launch);
+      case 2: return LaunchRegsOne<256, K, 2>(r, stream, launch);
+      case 4: return LaunchRegsOne<256, K, 4>(r, stream, launch);
+      default: return false;
+    }
+  }
+  if (plan.threads == 1024) {
+    switch (plan.vpt) {
+      case 2: return LaunchRegsOne<1024, K, 2>(r, stream, launch);
+      case 4: return LaunchRegsOne<1024, K, 4>(r, stream, launch);
+      case 6: return LaunchRegsOne<1024, K, 6>(r, stream, launch);
+      case 8: return LaunchRegsOne<1024, K, 8>(r, stream, launch);
+      case 10: return LaunchRegsOne<1024, K, 10>(r, stream, launch);
+      case 12: return LaunchRegsOne<1024, K, 12>(r, stream, launch);
+      default: return false;
+    }
+  }
+  return false;
+}
+
+bool DispatchRegs(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
+  switch (r.k) {
+    case 3: return LaunchRegsK<3>(r, plan, stream, launch);
+    case 4: return LaunchRegsK<4>(r, plan, stream, launch);
+    case 5: return LaunchRegsK<5>(r, plan, stream, launch);
+    case 6: return LaunchRegsK<6>(r, plan, stream, launch);
+    case 7: return LaunchRegsK<7>(r, plan, stream, launch);
+    case 8: return LaunchRegsK<8>(r, plan, stream, launch);
+    default: return false;
+  }
+}
+
 bool DispatchK2(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
   switch (plan.threads) {
     case 64: return LaunchK2Vpt<64>(r, plan, stream, launch);
@@ -366,9 +595,29 @@ bool DescentSupported(const DescentPlan& plan, int wide) {
       RunDev dummy{};
       return !wide && DispatchK2(dummy, plan, nullptr, false);
     }
+    case Descent::kRegisters: {
+      RunDev dummy{};
+      dummy.k = plan.k;
+      return !wide && DispatchRegs(dummy, plan, nullptr, false);
+    }
     default:
       return false;
   }
+}
+
+int RegistersVpt(int n, int k, int* threads) {
+  if (k < 3 || k > 8) return 0;
+  if (n <= 1024) {
+    *threads = 256;
+    for (int v : {1, 2, 4}) {
+      if (v * 256 >= n) return v;
+    }
+  }
+  *threads = 1024;
+  for (int v : {2, 4, 6, 8, 10, 12}) {
+    if (v * 1024 >= n) return RegsFit(k, v) ? v : 0;
+  }
+  return 0;
 }
 
 void LaunchDescent(const RunDev& r, const DescentPlan& plan, int wide, hipStream_t stream) {
@@ -383,6 +632,9 @@ void LaunchDescent(const RunDev& r, const DescentPlan& plan, int wide, hipStream
       break;
     case Descent::kK2Registers:
       DispatchK2(r, plan, stream, true);
+      break;
+    case Descent::kRegisters:
+      DispatchRegs(r, plan, stream, true);
       break;
     default:
       break;

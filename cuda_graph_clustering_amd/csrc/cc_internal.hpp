@@ -84,10 +84,18 @@ constexpr int kRecordIteration = 1;  // offer `next` after a descent, then stall
 constexpr int kRecordBatch = 2;      // Neighborhood batch: offer `next`, add moves, strict < keeps earlier starts
 
 // Descent kinds (values match CC_DESCENT_* of cc_hip.h).
-enum class Descent : int { kGenericGlobal = 0, kGenericLds = 1, kK2Registers = 2, kK2Wide = 3, kK3Wide = 4 };
+enum class Descent : int {
+  kGenericGlobal = 0,
+  kGenericLds = 1,
+  kK2Registers = 2,
+  kK2Wide = 3,
+  kK3Wide = 4,
+  kRegisters = 5,  // k in [3, 8], narrow, state in registers
+};
 
 struct DescentPlan {
   Descent descent = Descent::kGenericGlobal;
+  int k = 0;
   int threads = 256;
   int vpt = 0;             // vertices per thread (k = 2 registers)
   bool rows_in_lds = false;  // k = 2 registers: the whole adjacency staged in LDS
@@ -103,5 +111,7 @@ bool DescentSupported(const DescentPlan& plan, int wide);
 size_t DescentLdsLimit();
 // Dynamic LDS of the generic descent with G and labels in LDS.
 size_t GenericLdsBytes(int n, int k, int wide);
+// Vertices per thread of the register descent for (n, k), and its threads per block; 0 if no instance fits.
+int RegistersVpt(int n, int k, int* threads);
 
 }  // namespace cc::hip

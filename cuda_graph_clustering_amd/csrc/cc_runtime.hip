@@ -140,8 +140,19 @@ int GenericThreads(int n) {
 DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, int wide) {
   const int n = static_cast<int>(n_);
   DescentPlan plan;
+  plan.k = k;
   plan.blocks = p.population;
   plan.threads = GenericThreads(n);
+  if (p.ls_kernel == CC_LS_AUTO && !wide && k >= 3) {
+    int threads = 0;
+    const int vpt = RegistersVpt(n, k, &threads);
+    if (vpt > 0) {
+      plan.descent = Descent::kRegisters;
+      plan.threads = threads;
+      plan.vpt = vpt;
+      return plan;
+    }
+  }
   const size_t lds = GenericLdsBytes(n, k, wide);
   const bool fits = lds <= DescentLdsLimit();
   if (p.ls_kernel == CC_LS_SHARED && !fits) {
@@ -330,6 +341,7 @@ const char* DescentName(Descent d) {
     case Descent::kK2Registers: return "k = 2, registers";
     case Descent::kK2Wide: return "k = 2, packed words";
     case Descent::kK3Wide: return "k = 3, packed words";
+    case Descent::kRegisters: return "k = 3..8, registers";
   }
   return "?";
 }

@@ -82,6 +82,51 @@ def test_c1_full_config(cga, orc, cpu_threads):
     assert got["descent_name"] == "k2-registers"
 
 
+@pytest.fixture(scope="module")
+def c2_graph(cga):
+    """SBM n=10000 in 5 equal blocks, p_in 0.4, p_out 0.05, graph-seed 2."""
+    return cga.sbm(10000, 5, 0.4, 0.05, 2)
+
+
+@pytest.mark.parametrize("ls_kernel", ["auto", "shared"])
+def test_c2_full_size_trajectory(cga, orc, cpu_threads, c2_graph, ls_kernel):
+    """The C2 graph at full size (n=10000, k=5) bit-exact against the oracle; population 16 and 2 iterations keep
+    the CPU side to seconds."""
+    got, want = run_both(cga, orc, c2_graph, cpu_threads, k=5, population=16, iterations=2, early_stop=2, gww=0.1,
+                         seed=42, ls_kernel=ls_kernel)
+    check_same(got, want, f"C2 pop16 {ls_kernel}")
+
+
+def local_optimum_violations(graph, labels, k):
+    """Number of vertices with an improving single move, from G recomputed with numpy (no oracle code)."""
+    adj = np.unpackbits(graph.bits.view(np.uint8), axis=1, bitorder="little")[:, : graph.n].astype(np.float32)
+    z = np.zeros((graph.n, k), dtype=np.float32)
+    z[np.arange(graph.n), labels] = 1.0
+    g = (adj @ z).astype(np.int64)  # exact: counts < 2^24
+    sizes = np.bincount(labels, minlength=k).astype(np.int64)
+    own = g[np.arange(graph.n), labels]
+    delta = (sizes[None, :] - sizes[labels][:, None] + 1) - 2 * (g - own[:, None])
+    delta[np.arange(graph.n), labels] = 1
+    return int((delta < 0).any(axis=1).sum()), g, sizes
+
+
+def test_c2_full_config_properties(cga, c2_graph):
+    """C2 at its full size (pop 128, 200 iterations): properties that hold for the exact trajectory at any size —
+    the reported f is the recount of the labels, the labels are a local optimum, the record never increases, and
+    a second run (a fresh allocation) reproduces the first bit for bit."""
+    kw = dict(k=5, population=128, iterations=200, early_stop=200, gww=0.1, seed=42)
+    a = cga.solve_pbils(c2_graph, **kw)
+    b = cga.solve_pbils(c2_graph, **kw)
+    assert a["iterations_done"] == 200 and a["local_searches"] == 200 * 128
+    assert a["objective"] == cga.objective_bitwise(c2_graph, a["labels"])
+    violations, g, sizes = local_optimum_violations(c2_graph, a["labels"], 5)
+    assert violations == 0
+    assert (np.diff(a["trace"]) <= 0).all()
+    for key in KEYS:
+        assert a[key] == b[key]
+    assert (a["labels"] == b["labels"]).all() and (a["trace"] == b["trace"]).all()
+
+
 @pytest.mark.parametrize("ls_kernel", ["auto", "global", "shared"])
 def test_golden_gpu_csv_rows(cga, cpu_threads, ls_kernel):
     """docs/data/gpu.csv of the reference: n=1000, k=3, pop 128, 20 iterations, gww 0, graph-seed 7, seeds 6-8."""
