@@ -33,12 +33,18 @@ CONFIGS = {
                         "seed=42, early stop off",
                gen=("sbm", 10000, 5, 0.4, 0.05, 2), k=5, population=128, gww=0.1, seed=42, iterations=200,
                cpu_iterations=1),
+    "c5": dict(workload="Chung-Lu n=100000, degree exponent 2.5, mean degree 64, graph-seed 4; k=3, pop=128, "
+                        "gww=0.1, seed=42, early stop off (wide path, int32 G, packed k=3 descent)",
+               gen=("cl", 100000, 2.5, 64.0, 4), k=3, population=128, gww=0.1, seed=42, iterations=20,
+               cpu_iterations=1),
 }
 
 
 def make_graph(cga, gen):
     if gen[0] == "er":
         return cga.erdos_renyi(gen[1], gen[2], gen[3])
+    if gen[0] == "cl":
+        return cga.chung_lu(gen[1], gen[2], gen[3], gen[4])
     return cga.sbm(gen[1], gen[2], gen[3], gen[4], gen[5])
 
 
@@ -53,18 +59,40 @@ def descent_bytes_per_move(n, k, descent):
     return k * n * gbytes + n + row
 
 
-def cpu_baseline(cfg, graph, threads, iterations):
+def reference_solver():
     import oracle
 
     if not oracle.reference_available():
-        return None, None
-    ref = oracle.Reference()
+        return None
+    return oracle.Reference()
+
+
+def same_result(a, b):
+    return (a["objective"] == b["objective"] and a["accepted_moves"] == b["accepted_moves"] and
+            a["iterations_done"] == b["iterations_done"] and bool((a["labels"] == b["labels"]).all()))
+
+
+def cpu_sample(cfg, graph, dg, threads):
+    """The steady-state iteration on the CPU: reference solves of m and m + 1 iterations (m = cpu_iterations),
+    differenced on the solver's own clock; the GPU solves the same two prefixes and must match both bit for bit."""
+    import oracle
+
+    ref = reference_solver()
+    if ref is None:
+        return None
     og = oracle.Graph(graph.bits, graph.n, graph.edges)
-    t0 = time.perf_counter()
-    res = ref.solve_pbils(og, k=cfg["k"], population=cfg["population"], iterations=iterations, early_stop=iterations,
-                          gww=cfg["gww"], seed=cfg["seed"], threads=threads)
-    wall = time.perf_counter() - t0
-    return res, wall
+    common = dict(k=cfg["k"], population=cfg["population"], gww=cfg["gww"], seed=cfg["seed"])
+    m = cfg["cpu_iterations"]
+    out = {"iterations": m, "bit_exact": True}
+    for it in (m, m + 1):
+        cpu = ref.solve_pbils(og, iterations=it, early_stop=it, threads=threads, **common)
+        gpu = dg.solve_pbils(iterations=it, early_stop=it, **common)
+        out[f"cpu_{it}"] = cpu["seconds"]
+        out[f"gpu_{it}"] = gpu["seconds"]
+        out["bit_exact"] &= same_result(cpu, gpu)
+    out["cpu_iteration_s"] = out[f"cpu_{m + 1}"] - out[f"cpu_{m}"]
+    out["gpu_iteration_s"] = out[f"gpu_{m + 1}"] - out[f"gpu_{m}"]
+    return out
 
 
 def main():
@@ -94,19 +122,7 @@ def main():
         t0 = time.perf_counter()
         res = dg.solve_pbils(iterations=steps, early_stop=steps, profile=True, **common)
         wall = time.perf_counter() - t0
-        sample = None
-        if not args.no_cpu:
-            cpu_it = cfg["cpu_iterations"]
-            t1 = time.perf_counter()
-            gpu_sample = dg.solve_pbils(iterations=cpu_it, early_stop=cpu_it, **common)
-            gpu_sample_wall = time.perf_counter() - t1
-            cpu_res, cpu_wall = cpu_baseline(cfg, graph, args.cpu_threads, cpu_it)
-            if cpu_res is not None:
-                same = (cpu_res["objective"] == gpu_sample["objective"] and
-                        cpu_res["accepted_moves"] == gpu_sample["accepted_moves"] and
-                        (cpu_res["labels"] == gpu_sample["labels"]).all())
-                sample = dict(iterations=cpu_it, cpu_seconds=cpu_res["seconds"], cpu_wall=cpu_wall,
-                              gpu_seconds=gpu_sample["seconds"], gpu_wall=gpu_sample_wall, bit_exact=bool(same))
+        sample = None if args.no_cpu else cpu_sample(cfg, graph, dg, args.cpu_threads)
 
     n, k = graph.n, cfg["k"]
     iters = res["iterations_done"]
@@ -150,17 +166,19 @@ def main():
         },
     }
     if sample is not None:
-        cpu_rate = sample["iterations"] / sample["cpu_seconds"]
+        m = sample["iterations"]
+        cpu_rate = 1.0 / sample["cpu_iteration_s"]
         out["cpu_baseline"] = {
             "value": round(cpu_rate, 5),
             "unit": "iter/s",
             "cores": args.cpu_threads,
             "kind": "reference",
-            "sample": f"the first {sample['iterations']} PBILS iteration(s) of the same workload from the random "
-                      f"initial population (init and first descents included), reference SolveCpu with "
-                      f"{args.cpu_threads} threads, its own clock",
-            "gpu_same_sample_iter_per_s": round(sample["iterations"] / sample["gpu_seconds"], 3),
-            "gpu_over_cpu_same_sample": round(sample["cpu_seconds"] / sample["gpu_seconds"], 1),
+            "sample": f"iteration {m + 1} of the same workload (tournament, 128 descents from perturbed optima, "
+                      f"record, GWW, perturb, rebuild): reference SolveCpu with {args.cpu_threads} threads, "
+                      f"solves of {m + 1} and {m} iterations differenced on its own clock",
+            "cpu_seconds": {str(m): round(sample[f"cpu_{m}"], 3), str(m + 1): round(sample[f"cpu_{m + 1}"], 3)},
+            "gpu_same_iteration_s": round(sample["gpu_iteration_s"], 5),
+            "gpu_over_cpu": round(value / cpu_rate, 1),
             "bit_exact": sample["bit_exact"],
         }
     else:

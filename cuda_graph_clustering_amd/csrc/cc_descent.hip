@@ -464,6 +464,269 @@ __global__ void __launch_bounds__(T) KernelDescentRegs(RunDev r) {
   }
 }
 
+// Vertices per thread whose loads a wide-path scan keeps in flight at once.
+constexpr int kWideUnroll = 8;
+
+// Wide path, k = 2: one 32-bit word per vertex, w = 2 e + label with e = G1 - G0, kept in the slot's row G[0]
+// during the descent (reference algebra cuda/cc_local_search_k2.cuh:251-254): delta = sv - w for label 0 and
+// w + 1 - sv for label 1; a neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored
+// from w and the degrees at the end.
+template <int T>
+__global__ void __launch_bounds__(T) KernelDescentK2Wide(RunDev r) {
+  Ctl* ctl = r.ctl;
+  if (ctl->stopped) return;
+  constexpr int kWaves = T / kWave;
+  const int s = blockIdx.x;
+  const int n = r.n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = WaveIndex();
+  __shared__ uint64_t partial[2][kWaves];
+  __shared__ int32_t src_sh;
+  if (tid == 0) {
+    src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
+  }
+  __syncthreads();
+  const int src = src_sh;
+  const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 2 * n;
+  const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
+  int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
+  int32_t* state = g_out;  // row G[0] of the target slot
+  int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
+  const int64_t f_start = r.pop.f[src];
+
+  int sv = n - 2 * n0 + 1;
+  int best_d = 0, best_u = -1, best_w = 0;
+  for (int base = wave * kWave; base < n; base += T) {
+    const int u = base + lane;
+    if (u < n) {
+      const int w = 2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1);
+      state[u] = w;
+      const int d = (w & 1) ? w + 1 - sv : sv - w;
+      if (d < best_d) {
+        best_d = d;
+        best_u = u;
+        best_w = w;
+      }
+    }
+  }
+  uint64_t key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
+  int64_t objective = f_start;
+  int64_t applied = 0;
+  const int64_t cap = MoveCap(f_start, n);
+  int parity = 0;
+  const ConstWords bits = (ConstWords)r.bits;
+  while (true) {
+    const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
+    parity ^= 1;
+    if (best == kNoMove) break;
+    const int delta = static_cast<int>(static_cast<uint32_t>(best >> 32)) - kDeltaOffset;
+    const uint32_t mv = static_cast<uint32_t>(best);
+    const int v = static_cast<int>(mv >> 12);
+    const int from = static_cast<int>((mv >> 6) & 63u);
+    n0 += from ? 1 : -1;
+    sv = n - 2 * n0 + 1;
+    const int shift = from ? -4 : 4;
+    if (v % T == tid) state[v] ^= 1;
+    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
+    best_d = 0;
+    best_u = -1;
+    for (int base0 = wave * kWave; base0 < n; base0 += T * kWideUnroll) {
+      uint64_t words[kWideUnroll];
+      int ws[kWideUnroll];
+#pragma unroll
+      for (int i = 0; i < kWideUnroll; ++i) {
+        const int base = base0 + i * T;
+        const int u = base + lane;
+        words[i] = base < n ? row[base >> 6] : 0ull;
+        ws[i] = u < n ? state[u] : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kWideUnroll; ++i) {
+        const int u = base0 + i * T + lane;
+        if (u < n) {
+          int w = ws[i];
+          if ((words[i] >> lane) & 1ull) {
+            w += shift;
+            state[u] = w;
+          }
+          const int d = (w & 1) ? w + 1 - sv : sv - w;
+          if (d < best_d) {
+            best_d = d;
+            best_u = u;
+            best_w = w;
+          }
+        }
+      }
+    }
+    key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
+    objective += delta;
+    if (++applied >= cap) {
+      if (tid == 0) {
+        ctl->error = 1;
+        ctl->stopped = 1;
+      }
+      break;
+    }
+  }
+  uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
+  for (int u = tid; u < n; u += T) {
+    const int w = state[u];
+    const int e = w >> 1;
+    const int degree = r.deg[u];
+    lab_out[u] = static_cast<uint8_t>(w & 1);
+    g_out[u] = (degree - e) / 2;
+    g_out[n + u] = (degree + e) / 2;
+  }
+  if (tid == 0) {
+    r.next.sizes[static_cast<size_t>(s) * 2] = n0;
+    r.next.sizes[static_cast<size_t>(s) * 2 + 1] = n - n0;
+    r.next.f[s] = objective;
+    r.moves[s] = applied;
+  }
+}
+
+// Wide path, k = 3: one 64-bit word per vertex, G0 | G1 << 20 | G2 << 40 | label << 62 (G < 2^20), in the run's
+// work buffer. A neighbour update is one 64-bit add of (1 << 20 to) - (1 << 20 from): G[from] >= 1 there and
+// G[to] + 1 <= deg, so no field borrows or carries. Best target of a vertex with label a: the smaller
+// h_c = s_c - 2 G_c of the two other clusters, ties to the smaller c; delta = h_c - h_a + 1 (the reference's
+// cuda/cc_local_search_k3.cuh:20-67 states the same algebra).
+constexpr int kK3Bits = 20;
+constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
+constexpr int kK3LabelShift = 62;
+
+template <int T>
+__global__ void __launch_bounds__(T) KernelDescentK3Wide(RunDev r, uint64_t* work) {
+  Ctl* ctl = r.ctl;
+  if (ctl->stopped) return;
+  constexpr int kWaves = T / kWave;
+  const int s = blockIdx.x;
+  const int n = r.n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = WaveIndex();
+  __shared__ uint64_t partial[2][kWaves];
+  __shared__ int32_t src_sh;
+  if (tid == 0) {
+    src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
+  }
+  __syncthreads();
+  const int src = src_sh;
+  const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
+  const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
+  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
+  uint64_t* state = work + static_cast<size_t>(s) * n;
+  int s0 = sizes_in[0], s1 = sizes_in[1], s2 = sizes_in[2];
+  const int64_t f_start = r.pop.f[src];
+
+  int best_d = 0, best_move = -1;
+  auto consider = [&](uint64_t w, int u) {
+    const int a = static_cast<int>(w >> kK3LabelShift);
+    const int h0 = s0 - 2 * static_cast<int>(w & kK3Mask);
+    const int h1 = s1 - 2 * static_cast<int>((w >> kK3Bits) & kK3Mask);
+    const int h2 = s2 - 2 * static_cast<int>((w >> (2 * kK3Bits)) & kK3Mask);
+    const int ha = a == 0 ? h0 : (a == 1 ? h1 : h2);
+    const int lo = a == 0 ? h1 : h0;  // the other cluster with the smaller index
+    const int hi = a == 2 ? h1 : h2;
+    const bool take_hi = hi < lo;
+    const int d = (take_hi ? hi : lo) - ha + 1;
+    if (d < best_d) {
+      best_d = d;
+      const int to = take_hi ? (a == 2 ? 1 : 2) : (a == 0 ? 1 : 0);
+      best_move = (u << 4) | (a << 2) | to;
+    }
+  };
+  auto key_of = [&]() {
+    return best_move < 0 ? kNoMove : PackMove(best_d, best_move >> 4, (best_move >> 2) & 3, best_move & 3);
+  };
+  for (int base = wave * kWave; base < n; base += T) {
+    const int u = base + lane;
+    if (u < n) {
+      const uint64_t w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
+                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
+                         static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
+      state[u] = w;
+      consider(w, u);
+    }
+  }
+  uint64_t key = key_of();
+  int64_t objective = f_start;
+  int64_t applied = 0;
+  const int64_t cap = MoveCap(f_start, n);
+  int parity = 0;
+  const ConstWords bits = (ConstWords)r.bits;
+  while (true) {
+    const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
+    parity ^= 1;
+    if (best == kNoMove) break;
+    const int delta = static_cast<int>(static_cast<uint32_t>(best >> 32)) - kDeltaOffset;
+    const uint32_t mv = static_cast<uint32_t>(best);
+    const int v = static_cast<int>(mv >> 12);
+    const int from = static_cast<int>((mv >> 6) & 63u);
+    const int to = static_cast<int>(mv & 63u);
+    s0 += (to == 0) - (from == 0);
+    s1 += (to == 1) - (from == 1);
+    s2 += (to == 2) - (from == 2);
+    const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
+    if (v % T == tid) {
+      state[v] = (state[v] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
+    }
+    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
+    best_d = 0;
+    best_move = -1;
+    // kWideUnroll vertices per thread in flight: all loads of a group are issued before any is consumed.
+    for (int base0 = wave * kWave; base0 < n; base0 += T * kWideUnroll) {
+      uint64_t words[kWideUnroll];
+      uint64_t ws[kWideUnroll];
+#pragma unroll
+      for (int i = 0; i < kWideUnroll; ++i) {
+        const int base = base0 + i * T;
+        const int u = base + lane;
+        words[i] = base < n ? row[base >> 6] : 0ull;
+        ws[i] = u < n ? state[u] : 0ull;
+      }
+#pragma unroll
+      for (int i = 0; i < kWideUnroll; ++i) {
+        const int u = base0 + i * T + lane;
+        if (u < n) {
+          uint64_t w = ws[i];
+          if ((words[i] >> lane) & 1ull) {
+            w += inc;
+            state[u] = w;
+          }
+          consider(w, u);
+        }
+      }
+    }
+    key = key_of();
+    objective += delta;
+    if (++applied >= cap) {
+      if (tid == 0) {
+        ctl->error = 1;
+        ctl->stopped = 1;
+      }
+      break;
+    }
+  }
+  uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
+  int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
+  for (int u = tid; u < n; u += T) {
+    const uint64_t w = state[u];
+    g_out[u] = static_cast<int32_t>(w & kK3Mask);
+    g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
+    g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
+    lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
+  }
+  if (tid == 0) {
+    int32_t* sizes = r.next.sizes + static_cast<size_t>(s) * 3;
+    sizes[0] = s0;
+    sizes[1] = s1;
+    sizes[2] = s2;
+    r.next.f[s] = objective;
+    r.moves[s] = applied;
+  }
+}
+
 constexpr size_t kLdsBytes = 160 * 1024;
 // Static LDS of the descent kernels (partials, sizes, source slot), rounded up.
 constexpr size_t kStaticLdsReserve = 2048;
@@ -600,6 +863,10 @@ bool DescentSupported(const DescentPlan& plan, int wide) {
       dummy.k = plan.k;
       return !wide && DispatchRegs(dummy, plan, nullptr, false);
     }
+    case Descent::kK2Wide:
+      return wide && plan.k == 2 && (plan.threads == 256 || plan.threads == 1024);
+    case Descent::kK3Wide:
+      return wide && plan.k == 3 && (plan.threads == 256 || plan.threads == 1024);
     default:
       return false;
   }
@@ -635,6 +902,20 @@ void LaunchDescent(const RunDev& r, const DescentPlan& plan, int wide, hipStream
       break;
     case Descent::kRegisters:
       DispatchRegs(r, plan, stream, true);
+      break;
+    case Descent::kK2Wide:
+      if (plan.threads == 256) {
+        KernelDescentK2Wide<256><<<r.population, 256, 0, stream>>>(r);
+      } else {
+        KernelDescentK2Wide<1024><<<r.population, 1024, 0, stream>>>(r);
+      }
+      break;
+    case Descent::kK3Wide:
+      if (plan.threads == 256) {
+        KernelDescentK3Wide<256><<<r.population, 256, 0, stream>>>(r, r.work);
+      } else {
+        KernelDescentK3Wide<1024><<<r.population, 1024, 0, stream>>>(r, r.work);
+      }
       break;
     default:
       break;

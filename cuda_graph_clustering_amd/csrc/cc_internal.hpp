@@ -54,6 +54,7 @@ struct RunDev {
   int32_t* src;          // [P] GWW: which slot of `next` each slot of the new population starts from
   uint64_t* masks;       // [P][k][words] rebuild scratch
   int64_t* intra;        // [P] rebuild scratch: sum_v G[label v][v]
+  uint64_t* work;        // [P][n] packed vertex words of the k = 3 wide descent (null for other plans)
   uint64_t init_stream, select_stream, perturb_stream;
   int32_t tournament, copies, early_stop, iterations;
   float perturbation;

@@ -143,6 +143,18 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, int wide) 
   plan.k = k;
   plan.blocks = p.population;
   plan.threads = GenericThreads(n);
+  const size_t lds = GenericLdsBytes(n, k, wide);
+  const bool fits = lds <= DescentLdsLimit();
+  if (p.ls_kernel == CC_LS_SHARED && !fits) {
+    Fail(CC_HIP_EINVAL, "state needs %zu bytes of LDS, the device allows %zu", lds, DescentLdsLimit());
+  }
+  // Wide path: the packed k = 2 / k = 3 kernels unless the generic kernel is asked for
+  // (reference cuda/cc_pbils_gpu.cu:84-96).
+  if (wide && p.ls_kernel != CC_LS_GLOBAL && (k == 2 || k == 3)) {
+    plan.descent = k == 2 ? Descent::kK2Wide : Descent::kK3Wide;
+    plan.threads = n >= 8192 ? 1024 : 256;
+    return plan;
+  }
   if (p.ls_kernel == CC_LS_AUTO && !wide && k >= 3) {
     int threads = 0;
     const int vpt = RegistersVpt(n, k, &threads);
@@ -152,11 +164,6 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, int wide) 
       plan.vpt = vpt;
       return plan;
     }
-  }
-  const size_t lds = GenericLdsBytes(n, k, wide);
-  const bool fits = lds <= DescentLdsLimit();
-  if (p.ls_kernel == CC_LS_SHARED && !fits) {
-    Fail(CC_HIP_EINVAL, "state needs %zu bytes of LDS, the device allows %zu", lds, DescentLdsLimit());
   }
   if (p.ls_kernel == CC_LS_AUTO && k == 2 && !wide) {
     plan.descent = Descent::kK2Registers;
@@ -253,15 +260,17 @@ struct RunMemory {
   int32_t* order = nullptr;
   Ctl* host_ctl = nullptr;  // pinned
 
-  RunMemory(const DeviceGraph& g, const cc_pbils_params& p, int population, bool wide, bool has_next) {
+  RunMemory(const DeviceGraph& g, const cc_pbils_params& p, int population, bool wide, bool has_next,
+            const DescentPlan& plan) {
     const size_t n = g.n;
     const size_t k = static_cast<size_t>(p.k);
     const size_t P = static_cast<size_t>(population);
     const size_t gsize = wide ? 4 : 2;
     const size_t iterations = static_cast<size_t>(std::max(p.iterations, 1));
     const size_t pop_bytes = P * n + P * k * n * gsize + P * k * 4 + P * 8;
+    const size_t work_words = plan.descent == Descent::kK3Wide ? P * n : 0;
     const size_t bytes = pop_bytes * (has_next ? 2 : 1) + P * 8 * 2 + P * 4 * 2 + n + iterations * 8 +
-                         P * k * g.words * 8 + sizeof(Ctl) + 4096;
+                         P * k * g.words * 8 + work_words * 8 + sizeof(Ctl) + 4096;
     size_t free_bytes = 0, total_bytes = 0;
     CC_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
     if (bytes > free_bytes) {
@@ -296,6 +305,7 @@ struct RunMemory {
     order = memory.Allocate<int32_t>(P);
     r.masks = memory.Allocate<uint64_t>(P * k * g.words);
     r.intra = memory.Allocate<int64_t>(P);
+    r.work = memory.Allocate<uint64_t>(work_words);
     CC_HIP_CHECK(hipMemset(r.moves, 0, P * sizeof(int64_t)));
     CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(Ctl), hipHostMallocDefault));
   }
@@ -367,7 +377,7 @@ void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_
   if (!DescentSupported(plan, wide)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
   Stream stream;
   const hipStream_t s = stream.get();
-  RunMemory run(g, p, p.population, wide != 0, true);
+  RunMemory run(g, p, p.population, wide != 0, true, plan);
   RunDev& r = run.r;
   r.init_stream = StreamSeed(p.seed, kInitStream, 0);
   r.select_stream = StreamSeed(p.seed, kSelectStream, 0);
@@ -472,7 +482,7 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   if (!DescentSupported(plan, wide)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
   Stream stream;
   const hipStream_t s = stream.get();
-  RunMemory run(g, p, p.population, wide != 0, false);
+  RunMemory run(g, p, p.population, wide != 0, false, plan);
   RunDev& r = run.r;
   r.select = 0;
   r.iterations = 1;

@@ -97,6 +97,23 @@ def test_c2_full_size_trajectory(cga, orc, cpu_threads, c2_graph, ls_kernel):
     check_same(got, want, f"C2 pop16 {ls_kernel}")
 
 
+@pytest.fixture(scope="module")
+def wide_graph(cga):
+    """Chung-Lu n=40000 (> 32767: the wide path), degree exponent 2.5, mean degree 16."""
+    return cga.chung_lu(40000, 2.5, 16.0, 9)
+
+
+@pytest.mark.parametrize("k,ls_kernel", [(2, "auto"), (3, "auto"), (3, "global"), (4, "auto")])
+def test_wide_path_full_size_trajectory(cga, orc, cpu_threads, wide_graph, k, ls_kernel):
+    """n above the 16-bit range: int32 G, the packed k = 2 / k = 3 kernels (auto) or the generic global kernel."""
+    got, want = run_both(cga, orc, wide_graph, cpu_threads, k=k, population=4, iterations=2, early_stop=2,
+                         gww=0.25, seed=3, ls_kernel=ls_kernel)
+    check_same(got, want, f"wide k={k} {ls_kernel}")
+    expected = {(2, "auto"): "k2-wide", (3, "auto"): "k3-wide", (3, "global"): "generic-global",
+                (4, "auto"): "generic-global"}[(k, ls_kernel)]
+    assert got["descent_name"] == expected
+
+
 def local_optimum_violations(graph, labels, k):
     """Number of vertices with an improving single move, from G recomputed with numpy (no oracle code)."""
     adj = np.unpackbits(graph.bits.view(np.uint8), axis=1, bitorder="little")[:, : graph.n].astype(np.float32)
