@@ -73,6 +73,16 @@ class Stats(ctypes.Structure):
         return d
 
 
+class BatchRun(ctypes.Structure):
+    _fields_ = [
+        ("graph", ctypes.c_void_p),
+        ("params", Params),
+        ("labels_out", ctypes.POINTER(ctypes.c_int32)),
+        ("trace_out", ctypes.POINTER(ctypes.c_int64)),
+        ("stats", Stats),
+    ]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -99,6 +109,7 @@ ABI_SYMBOLS = [
     "cc_hip_graph_free",
     "cc_hip_graph_solve_pbils",
     "cc_hip_graph_solve_neighborhood",
+    "cc_hip_solve_batch",
     "cc_words64",
     "cc_graph_erdos_renyi",
     "cc_graph_sbm",
@@ -126,6 +137,8 @@ def _load():
                                              _i64p, ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_hip_graph_solve_neighborhood.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, _i32p,
                                                     ctypes.POINTER(Stats), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_solve_batch.argtypes = [ctypes.POINTER(BatchRun), ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_words64.argtypes = [ctypes.c_uint32]
     lib.cc_words64.restype = ctypes.c_uint32
     lib.cc_graph_erdos_renyi.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, _u64p, ctypes.c_int32]
@@ -293,6 +306,30 @@ class DeviceGraph:
                                                  labels.ctypes.data_as(_i32p), ctypes.byref(stats), err, 512)
         _check(rc, err)
         return _result(stats, labels, self.n)
+
+
+def solve_batch(device_graphs, params, use_graph=True):
+    """Many independent PBILS runs advanced together (cc_hip_solve_batch). device_graphs: DeviceGraph per run;
+    params: one dict of make_params() keywords per run. Returns (results, seconds); each result is the same dict
+    solve_pbils returns, bit-identical to the run's own solve."""
+    count = len(device_graphs)
+    runs = (BatchRun * count)()
+    keep = []
+    for i, (dg, kw) in enumerate(zip(device_graphs, params)):
+        p = make_params(**kw)
+        labels = np.zeros(max(dg.n, 1), dtype=np.int32)
+        trace = np.zeros(max(p.iterations, 1), dtype=np.int64)
+        keep.append((labels, trace, dg.n))
+        runs[i].graph = dg.handle
+        runs[i].params = p
+        runs[i].labels_out = labels.ctypes.data_as(_i32p)
+        runs[i].trace_out = trace.ctypes.data_as(_i64p)
+    seconds = ctypes.c_double(0.0)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.cc_hip_solve_batch(runs, count, int(bool(use_graph)), ctypes.byref(seconds), err, 512)
+    _check(rc, err)
+    results = [_result(runs[i].stats, keep[i][0], keep[i][2], keep[i][1]) for i in range(count)]
+    return results, seconds.value
 
 
 def device_info():

@@ -36,11 +36,9 @@ __device__ __forceinline__ uint64_t PackMove(int delta, int v, int from, int to)
 __host__ __device__ constexpr size_t Align16(size_t x) { return (x + 15) & ~static_cast<size_t>(15); }
 
 template <int T, typename GT, bool kLds>
-__global__ void __launch_bounds__(T) KernelDescentGeneric(RunDev r) {
+__device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  if (ctl->stopped) return;
   constexpr int kWaves = T / kWave;
-  const int s = blockIdx.x;
   const int n = r.n, k = r.k;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -171,12 +169,10 @@ __global__ void __launch_bounds__(T) KernelDescentGeneric(RunDev r) {
 constexpr uint32_t kNoMove32 = ~0u;
 
 template <int T, int kVpt, bool kRowsLds>
-__global__ void __launch_bounds__(T) KernelDescentK2(RunDev r) {
+__device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  if (ctl->stopped) return;
   constexpr int kWaves = T / kWave;
   static_assert(kVpt <= 32, "labels of a thread are packed into one 32-bit word");
-  const int s = blockIdx.x;
   const int n = r.n;
   const int words = r.words;
   const int tid = threadIdx.x;
@@ -300,14 +296,12 @@ using ConstWords = const __attribute__((address_space(4))) uint64_t*;
 // t_a = [a == to] - [a == from] (G change of a neighbour plus the change of K_a through s_a); t_a is fetched per
 // lane with ds_bpermute from a lane table.
 template <int T, int K, int kVpt>
-__global__ void __launch_bounds__(T) KernelDescentRegs(RunDev r) {
+__device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  if (ctl->stopped) return;
   constexpr int kWaves = T / kWave;
   constexpr int kOff = 1 << 16;
   constexpr int kIdleKey = 1 << 30;  // own key of a vertex beyond n: never improvable
   static_assert(K >= 3 && K <= 8, "register descent handles k in [3, 8]");
-  const int s = blockIdx.x;
   const int n = r.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -472,11 +466,9 @@ constexpr int kWideUnroll = 8;
 // w + 1 - sv for label 1; a neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored
 // from w and the degrees at the end.
 template <int T>
-__global__ void __launch_bounds__(T) KernelDescentK2Wide(RunDev r) {
+__device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  if (ctl->stopped) return;
   constexpr int kWaves = T / kWave;
-  const int s = blockIdx.x;
   const int n = r.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -596,11 +588,10 @@ constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
 constexpr int kK3LabelShift = 62;
 
 template <int T>
-__global__ void __launch_bounds__(T) KernelDescentK3Wide(RunDev r, uint64_t* work) {
+__device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  if (ctl->stopped) return;
+  uint64_t* work = r.work;
   constexpr int kWaves = T / kWave;
-  const int s = blockIdx.x;
   const int n = r.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -728,8 +719,52 @@ __global__ void __launch_bounds__(T) KernelDescentK3Wide(RunDev r, uint64_t* wor
 }
 
 constexpr size_t kLdsBytes = 160 * 1024;
-// Static LDS of the descent kernels (partials, sizes, source slot), rounded up.
+// Static LDS of the descent kernels (partials, sizes, source slot, claim), rounded up.
 constexpr size_t kStaticLdsReserve = 2048;
+
+// A persistent workgroup: claims items (run, slot) from the launch's queue until none is left and runs the descent
+// of each; items of stopped runs are skipped. The barrier before each claim keeps the shared state of the previous
+// item alive until every wave is done with it.
+template <typename Body>
+__device__ __forceinline__ void Persistent(const RunDev* runs, const ItemList& list, Body&& body) {
+  __shared__ int32_t claimed;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) claimed = atomicAdd(list.queue, 1);
+    __syncthreads();
+    const int index = claimed;
+    if (index >= list.count) break;
+    const Item it = list.items[index];
+    const RunDev& r = runs[it.run];
+    if (r.ctl->stopped) continue;
+    body(r, it.slot);
+  }
+}
+
+template <int T, typename GT, bool kLds>
+__global__ void __launch_bounds__(T) KernelDescentGeneric(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentGenericBody<T, GT, kLds>(r, s); });
+}
+
+template <int T, int kVpt, bool kRowsLds>
+__global__ void __launch_bounds__(T) KernelDescentK2(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentK2Body<T, kVpt, kRowsLds>(r, s); });
+}
+
+template <int T, int K, int kVpt>
+__global__ void __launch_bounds__(T) KernelDescentRegs(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentRegsBody<T, K, kVpt>(r, s); });
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentK2WideBody<T>(r, s); });
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentK3WideBody<T>(r, s); });
+}
 
 template <typename Kernel>
 void AllowLds(Kernel kernel, size_t bytes) {
@@ -739,47 +774,68 @@ void AllowLds(Kernel kernel, size_t bytes) {
   }
 }
 
-template <int T, typename GT>
-void LaunchGeneric(const RunDev& r, const DescentPlan& plan, hipStream_t stream) {
-  const int blocks = r.population;
-  if (plan.descent == Descent::kGenericLds) {
-    auto kernel = KernelDescentGeneric<T, GT, true>;
-    AllowLds(kernel, plan.lds_bytes);
-    kernel<<<blocks, T, plan.lds_bytes, stream>>>(r);
-  } else {
-    KernelDescentGeneric<T, GT, false><<<blocks, T, 0, stream>>>(r);
+// Launch context shared by the dispatchers below; `launch` false only asks whether the instance exists.
+struct Go {
+  const RunDev* runs;
+  const ItemList* list;
+  const DescentPlan* plan;
+  hipStream_t stream;
+  bool launch;
+
+  int Blocks() const { return plan->blocks > 0 ? plan->blocks : list->count; }
+
+  template <typename Kernel>
+  bool Run(Kernel kernel, int threads, size_t lds) const {
+    if (launch) {
+      AllowLds(kernel, lds);
+      kernel<<<Blocks(), threads, lds, stream>>>(runs, *list);
+    }
+    return true;
   }
+};
+
+template <int T, typename GT>
+bool GoGeneric(const Go& go) {
+  if (go.plan->descent == Descent::kGenericLds) return go.Run(KernelDescentGeneric<T, GT, true>, T, go.plan->lds_bytes);
+  return go.Run(KernelDescentGeneric<T, GT, false>, T, 0);
 }
 
 template <typename GT>
-void LaunchGenericAny(const RunDev& r, const DescentPlan& plan, hipStream_t stream) {
-  switch (plan.threads) {
-    case 256: LaunchGeneric<256, GT>(r, plan, stream); break;
-    case 512: LaunchGeneric<512, GT>(r, plan, stream); break;
-    default: LaunchGeneric<1024, GT>(r, plan, stream); break;
+bool GoGenericT(const Go& go) {
+  switch (go.plan->threads) {
+    case 256: return GoGeneric<256, GT>(go);
+    case 512: return GoGeneric<512, GT>(go);
+    case 1024: return GoGeneric<1024, GT>(go);
+    default: return false;
   }
 }
 
 template <int T, int kVpt>
-void LaunchK2(const RunDev& r, const DescentPlan& plan, hipStream_t stream) {
-  if (plan.rows_in_lds) {
-    auto kernel = KernelDescentK2<T, kVpt, true>;
-    AllowLds(kernel, plan.lds_bytes);
-    kernel<<<r.population, T, plan.lds_bytes, stream>>>(r);
-  } else {
-    KernelDescentK2<T, kVpt, false><<<r.population, T, 0, stream>>>(r);
-  }
+bool GoK2(const Go& go) {
+  if (go.plan->rows_in_lds) return go.Run(KernelDescentK2<T, kVpt, true>, T, go.plan->lds_bytes);
+  return go.Run(KernelDescentK2<T, kVpt, false>, T, 0);
 }
 
 template <int T>
-bool LaunchK2Vpt(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
-  switch (plan.vpt) {
-    case 1: if (launch) LaunchK2<T, 1>(r, plan, stream); return true;
-    case 2: if (launch) LaunchK2<T, 2>(r, plan, stream); return true;
-    case 4: if (launch) LaunchK2<T, 4>(r, plan, stream); return true;
-    case 8: if (launch) LaunchK2<T, 8>(r, plan, stream); return true;
-    case 16: if (launch) LaunchK2<T, 16>(r, plan, stream); return true;
-    case 32: if (launch) LaunchK2<T, 32>(r, plan, stream); return true;
+bool GoK2Vpt(const Go& go) {
+  switch (go.plan->vpt) {
+    case 1: return GoK2<T, 1>(go);
+    case 2: return GoK2<T, 2>(go);
+    case 4: return GoK2<T, 4>(go);
+    case 8: return GoK2<T, 8>(go);
+    case 16: return GoK2<T, 16>(go);
+    case 32: return GoK2<T, 32>(go);
+    default: return false;
+  }
+}
+
+bool GoK2Any(const Go& go) {
+  switch (go.plan->threads) {
+    case 64: return GoK2Vpt<64>(go);
+    case 128: return GoK2Vpt<128>(go);
+    case 256: return GoK2Vpt<256>(go);
+    case 512: return GoK2Vpt<512>(go);
+    case 1024: return GoK2Vpt<1024>(go);
     default: return false;
   }
 }
@@ -789,87 +845,100 @@ bool LaunchK2Vpt(const RunDev& r, const DescentPlan& plan, hipStream_t stream, b
 constexpr bool RegsFit(int k, int vpt) { return vpt * (k + 5) <= 112; }
 
 template <int T, int K, int kVpt>
-bool LaunchRegsOne(const RunDev& r, hipStream_t stream, bool launch) {
+bool GoRegsOne(const Go& go) {
   if constexpr (RegsFit(K, kVpt)) {
-    if (launch) KernelDescentRegs<T, K, kVpt><<<r.population, T, 0, stream>>>(r);
-    return true;
+    return go.Run(KernelDescentRegs<T, K, kVpt>, T, 0);
   } else {
     return false;
   }
 }
 
 template <int K>
-bool LaunchRegsK(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
-  if (plan.threads == 256) {
-    switch (plan.vpt) {
-      case 1: return LaunchRegsOne<256, K, 1>(r, stream, launch);
-      case 2: return LaunchRegsOne<256, K, 2>(r, stream, launch);
-      case 4: return LaunchRegsOne<256, K, 4>(r, stream, launch);
+bool GoRegsK(const Go& go) {
+  if (go.plan->threads == 256) {
+    switch (go.plan->vpt) {
+      case 1: return GoRegsOne<256, K, 1>(go);
+      case 2: return GoRegsOne<256, K, 2>(go);
+      case 4: return GoRegsOne<256, K, 4>(go);
       default: return false;
     }
   }
-  if (plan.threads == 1024) {
-    switch (plan.vpt) {
-      case 2: return LaunchRegsOne<1024, K, 2>(r, stream, launch);
-      case 4: return LaunchRegsOne<1024, K, 4>(r, stream, launch);
-      case 6: return LaunchRegsOne<1024, K, 6>(r, stream, launch);
-      case 8: return LaunchRegsOne<1024, K, 8>(r, stream, launch);
-      case 10: return LaunchRegsOne<1024, K, 10>(r, stream, launch);
-      case 12: return LaunchRegsOne<1024, K, 12>(r, stream, launch);
+  if (go.plan->threads == 1024) {
+    switch (go.plan->vpt) {
+      case 2: return GoRegsOne<1024, K, 2>(go);
+      case 4: return GoRegsOne<1024, K, 4>(go);
+      case 6: return GoRegsOne<1024, K, 6>(go);
+      case 8: return GoRegsOne<1024, K, 8>(go);
+      case 10: return GoRegsOne<1024, K, 10>(go);
+      case 12: return GoRegsOne<1024, K, 12>(go);
       default: return false;
     }
   }
   return false;
 }
 
-bool DispatchRegs(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
-  switch (r.k) {
-    case 3: return LaunchRegsK<3>(r, plan, stream, launch);
-    case 4: return LaunchRegsK<4>(r, plan, stream, launch);
-    case 5: return LaunchRegsK<5>(r, plan, stream, launch);
-    case 6: return LaunchRegsK<6>(r, plan, stream, launch);
-    case 7: return LaunchRegsK<7>(r, plan, stream, launch);
-    case 8: return LaunchRegsK<8>(r, plan, stream, launch);
+bool GoRegs(const Go& go) {
+  switch (go.plan->k) {
+    case 3: return GoRegsK<3>(go);
+    case 4: return GoRegsK<4>(go);
+    case 5: return GoRegsK<5>(go);
+    case 6: return GoRegsK<6>(go);
+    case 7: return GoRegsK<7>(go);
+    case 8: return GoRegsK<8>(go);
     default: return false;
   }
 }
 
-bool DispatchK2(const RunDev& r, const DescentPlan& plan, hipStream_t stream, bool launch) {
-  switch (plan.threads) {
-    case 64: return LaunchK2Vpt<64>(r, plan, stream, launch);
-    case 128: return LaunchK2Vpt<128>(r, plan, stream, launch);
-    case 256: return LaunchK2Vpt<256>(r, plan, stream, launch);
-    case 512: return LaunchK2Vpt<512>(r, plan, stream, launch);
-    case 1024: return LaunchK2Vpt<1024>(r, plan, stream, launch);
-    default: return false;
+template <template <int> class Wide>
+bool GoWide(const Go& go) {
+  if (go.plan->threads == 256) return go.Run(Wide<256>::kernel, 256, 0);
+  if (go.plan->threads == 1024) return go.Run(Wide<1024>::kernel, 1024, 0);
+  return false;
+}
+
+template <int T>
+struct WideK2 {
+  static constexpr auto kernel = KernelDescentK2Wide<T>;
+};
+template <int T>
+struct WideK3 {
+  static constexpr auto kernel = KernelDescentK3Wide<T>;
+};
+
+bool Dispatch(const Go& go) {
+  const DescentPlan& p = *go.plan;
+  switch (p.descent) {
+    case Descent::kGenericGlobal:
+    case Descent::kGenericLds:
+      return p.wide ? GoGenericT<int32_t>(go) : GoGenericT<int16_t>(go);
+    case Descent::kK2Registers:
+      return !p.wide && GoK2Any(go);
+    case Descent::kRegisters:
+      return !p.wide && GoRegs(go);
+    case Descent::kK2Wide:
+      return p.wide && p.k == 2 && GoWide<WideK2>(go);
+    case Descent::kK3Wide:
+      return p.wide && p.k == 3 && GoWide<WideK3>(go);
   }
+  return false;
 }
 
 }  // namespace
 
 size_t DescentLdsLimit() { return kLdsBytes - kStaticLdsReserve; }
 
-bool DescentSupported(const DescentPlan& plan, int wide) {
-  switch (plan.descent) {
-    case Descent::kGenericGlobal:
-    case Descent::kGenericLds:
-      return plan.threads == 256 || plan.threads == 512 || plan.threads == 1024;
-    case Descent::kK2Registers: {
-      RunDev dummy{};
-      return !wide && DispatchK2(dummy, plan, nullptr, false);
-    }
-    case Descent::kRegisters: {
-      RunDev dummy{};
-      dummy.k = plan.k;
-      return !wide && DispatchRegs(dummy, plan, nullptr, false);
-    }
-    case Descent::kK2Wide:
-      return wide && plan.k == 2 && (plan.threads == 256 || plan.threads == 1024);
-    case Descent::kK3Wide:
-      return wide && plan.k == 3 && (plan.threads == 256 || plan.threads == 1024);
-    default:
-      return false;
-  }
+bool DescentSupported(const DescentPlan& plan) {
+  const ItemList none{nullptr, 0, nullptr};
+  return Dispatch(Go{nullptr, &none, &plan, nullptr, false});
+}
+
+void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan& plan, hipStream_t stream) {
+  if (items.count == 0) return;
+  Dispatch(Go{runs, &items, &plan, stream, true});
+}
+
+size_t GenericLdsBytes(int n, int k, int wide) {
+  return Align16((wide ? 4 : 2) * static_cast<size_t>(k) * n) + Align16(static_cast<size_t>(n));
 }
 
 int RegistersVpt(int n, int k, int* threads) {
@@ -885,45 +954,6 @@ int RegistersVpt(int n, int k, int* threads) {
     if (v * 1024 >= n) return RegsFit(k, v) ? v : 0;
   }
   return 0;
-}
-
-void LaunchDescent(const RunDev& r, const DescentPlan& plan, int wide, hipStream_t stream) {
-  switch (plan.descent) {
-    case Descent::kGenericGlobal:
-    case Descent::kGenericLds:
-      if (wide) {
-        LaunchGenericAny<int32_t>(r, plan, stream);
-      } else {
-        LaunchGenericAny<int16_t>(r, plan, stream);
-      }
-      break;
-    case Descent::kK2Registers:
-      DispatchK2(r, plan, stream, true);
-      break;
-    case Descent::kRegisters:
-      DispatchRegs(r, plan, stream, true);
-      break;
-    case Descent::kK2Wide:
-      if (plan.threads == 256) {
-        KernelDescentK2Wide<256><<<r.population, 256, 0, stream>>>(r);
-      } else {
-        KernelDescentK2Wide<1024><<<r.population, 1024, 0, stream>>>(r);
-      }
-      break;
-    case Descent::kK3Wide:
-      if (plan.threads == 256) {
-        KernelDescentK3Wide<256><<<r.population, 256, 0, stream>>>(r, r.work);
-      } else {
-        KernelDescentK3Wide<1024><<<r.population, 1024, 0, stream>>>(r, r.work);
-      }
-      break;
-    default:
-      break;
-  }
-}
-
-size_t GenericLdsBytes(int n, int k, int wide) {
-  return Align16((wide ? 4 : 2) * static_cast<size_t>(k) * n) + Align16(static_cast<size_t>(n));
 }
 
 }  // namespace cc::hip

@@ -1,4 +1,8 @@
-// cc_internal.hpp — definitions shared by the kernels (cc_kernels.hip) and the host runtime (cc_runtime.hip).
+// cc_internal.hpp — definitions shared by the kernels (cc_kernels.hip, cc_descent.hip) and the host runtime
+// (cc_runtime.hip, cc_batch.hip).
+//
+// Every kernel works on a batch: it reads the run descriptors (RunDev) of many independent PBILS runs from a device
+// array and finds its run and slot through an item table. A single solve is a batch of one run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,7 +26,7 @@ struct Ctl {
   int32_t perturb_iteration;  // iteration index of the pending perturbation
   int32_t stall;              // iterations without a new record
   int32_t iterations_done;
-  int32_t stopped;            // set once the run is over: every later kernel of the run returns at once
+  int32_t stopped;            // set once the run is over: every later kernel skips the run
   int32_t have_record;
   int32_t rec_slot;
   int32_t error;              // a descent exceeded MoveCap(): the run is stopped and reported as failed
@@ -39,7 +43,7 @@ struct PopDev {
   int64_t* f;       // [P]
 };
 
-// Everything the kernels of one run see; passed by value.
+// One run (one graph, one parameter set). Lives in a device array; kernels read it through a pointer.
 struct RunDev {
   const uint64_t* bits;  // [n][words]
   int32_t* deg;          // [n], written once by KernelDegrees
@@ -52,6 +56,7 @@ struct RunDev {
   uint8_t* rec_labels;   // [n]
   int64_t* trace;        // [iterations] record after each iteration
   int32_t* src;          // [P] GWW: which slot of `next` each slot of the new population starts from
+  int32_t* order;        // [P] GWW ranking scratch
   uint64_t* masks;       // [P][k][words] rebuild scratch
   int64_t* intra;        // [P] rebuild scratch: sum_v G[label v][v]
   uint64_t* work;        // [P][n] packed vertex words of the k = 3 wide descent (null for other plans)
@@ -60,6 +65,20 @@ struct RunDev {
   float perturbation;
   int32_t do_perturb;  // k >= 2 and perturbation > 0 (src/cc_pbils_cpu.cpp:241)
   int32_t select;      // 1: the descent of slot s starts from Tournament(pop); 0: from pop[s] itself
+  int32_t wide;        // 32-bit G
+};
+
+// A slot of a run: the unit of work of the slot-level kernels and of the descent.
+struct Item {
+  int32_t run;
+  int32_t slot;
+};
+
+// The items of one launch, with the persistent-grid work counter of the descent.
+struct ItemList {
+  const Item* items;
+  int32_t count;
+  int32_t* queue;  // descent: next item to claim (reset in-stream by KernelResetQueues)
 };
 
 // Every accepted move lowers f by at least 1, so a descent ends after at most f moves; in practice it takes well
@@ -70,19 +89,33 @@ __host__ __device__ inline int64_t MoveCap(int64_t f_start, int n) {
   return f_start < practical ? f_start + 1 : practical;
 }
 
-// Kernel launchers (cc_kernels.hip). All take the stream of the run.
-void LaunchInitLabels(const RunDev& r, hipStream_t stream);
-void LaunchDegrees(const RunDev& r, hipStream_t stream);
-void LaunchRebuild(const RunDev& r, int wide, hipStream_t stream);
-void LaunchRecord(const RunDev& r, int mode, int read_next, int32_t* order, hipStream_t stream);
-void LaunchPerturb(const RunDev& r, hipStream_t stream);
-void LaunchNeighborhoodLabels(const RunDev& r, int first, int count, hipStream_t stream);
-void LaunchResetCtl(const RunDev& r, hipStream_t stream);
+// ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
+
+void LaunchResetCtl(const RunDev* runs, int nruns, hipStream_t stream);
+void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream);
+void LaunchDegrees(const RunDev* run, int n, hipStream_t stream);
+void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream);
+// G, sizes and f of `pop` for the items; `pairs` lists (run, first slot) of slot pairs of the same run, grouped by
+// the cluster chunk (kc = 2, 4 or 8) and G width of their run.
+struct RebuildGroup {
+  ItemList pairs;
+  int kc;
+  int wide;
+  int max_k;
+};
+void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream);
+void LaunchRebuildG(const RunDev* runs, const RebuildGroup& group, int max_n, hipStream_t stream);
+void LaunchFinalize(const RunDev* runs, const ItemList& items, hipStream_t stream);
+void LaunchRecord(const RunDev* runs, int nruns, int mode, int read_next, hipStream_t stream);
+void LaunchPerturb(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream);
+void LaunchNeighborhoodLabels(const RunDev* run, int n, int first, int count, hipStream_t stream);
 
 // Record kernel modes.
 constexpr int kRecordInitial = 0;    // offer the population after init (no iteration bookkeeping)
 constexpr int kRecordIteration = 1;  // offer `next` after a descent, then stall / stop / GWW ranking
 constexpr int kRecordBatch = 2;      // Neighborhood batch: offer `next`, add moves, strict < keeps earlier starts
+
+// ---- descent (cc_descent.hip) --------------------------------------------------------------------------------
 
 // Descent kinds (values match CC_DESCENT_* of cc_hip.h).
 enum class Descent : int {
@@ -98,21 +131,24 @@ struct DescentPlan {
   Descent descent = Descent::kGenericGlobal;
   int k = 0;
   int threads = 256;
-  int vpt = 0;             // vertices per thread (k = 2 registers)
+  int vpt = 0;               // vertices per thread (register kernels)
   bool rows_in_lds = false;  // k = 2 registers: the whole adjacency staged in LDS
-  size_t lds_bytes = 0;    // dynamic LDS
-  int blocks = 0;
+  size_t lds_bytes = 0;      // dynamic LDS
+  int wide = 0;
+  int blocks = 0;            // persistent workgroups
 };
 
-// Descent launcher; `wide` selects 32-bit G.
-void LaunchDescent(const RunDev& r, const DescentPlan& plan, int wide, hipStream_t stream);
-// Whether the descent kernels exist for this plan (template instance present).
-bool DescentSupported(const DescentPlan& plan, int wide);
-// Largest dynamic LDS the descent kernels may request.
+// Same kernel instance and launch shape: the items of such plans can share one launch.
+inline bool SameLaunch(const DescentPlan& a, const DescentPlan& b) {
+  return a.descent == b.descent && a.threads == b.threads && a.vpt == b.vpt && a.rows_in_lds == b.rows_in_lds &&
+         a.wide == b.wide && (a.descent != Descent::kRegisters || a.k == b.k);
+}
+
+// A persistent launch: plan.blocks workgroups claim the items of `items` through items.queue.
+void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan& plan, hipStream_t stream);
+bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
-// Dynamic LDS of the generic descent with G and labels in LDS.
 size_t GenericLdsBytes(int n, int k, int wide);
-// Vertices per thread of the register descent for (n, k), and its threads per block; 0 if no instance fits.
 int RegistersVpt(int n, int k, int* threads);
 
 }  // namespace cc::hip

@@ -1,9 +1,11 @@
 // cc_kernels.hip — population kernels of the PBILS path for gfx950: initial labels, G rebuild, record and GWW
 // ranking, perturbation, Neighborhood start partitions. The descent kernels live in cc_descent.hip.
 //
-// Every kernel of an iteration first reads ctl->stopped and returns when the run is over, so the host can queue
-// (or replay from a HIP graph) whole chunks of iterations without a round trip per iteration and still end on
-// exactly the iteration where the reference's early stop fires.
+// All kernels are batched: a block finds its run and slot through an item table (or, for the record, one block per
+// run), so one launch serves any number of independent runs of different n and k. Every kernel of an iteration
+// skips runs whose ctl->stopped is set, so the host can queue (or replay from a HIP graph) whole chunks of iterations
+// without a round trip per iteration and still end each run on exactly the iteration where the reference's early
+// stop fires.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -16,36 +18,45 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__global__ void KernelResetCtl(Ctl* ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    Ctl c{};
-    c.rec_f = -1;
-    *ctl = c;
-  }
+__global__ void KernelResetCtl(const RunDev* runs, int nruns) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nruns) return;
+  Ctl c{};
+  c.rec_f = -1;
+  *runs[i].ctl = c;
+}
+
+__global__ void KernelResetQueues(int32_t* queues, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) queues[i] = 0;
 }
 
 // labels[slot][v] = InitialLabel(init, slot * n + v, k)  (reference: KernelInitLabels, src/cc_pbils_cpu.cpp:198-205)
-__global__ void __launch_bounds__(kBlock) KernelInitLabels(RunDev r) {
+__global__ void __launch_bounds__(kBlock) KernelInitLabels(const RunDev* runs, const Item* items) {
+  const Item it = items[blockIdx.y];
+  const RunDev& r = runs[it.run];
   const int v = blockIdx.x * kBlock + threadIdx.x;
-  const int s = blockIdx.y;
   if (v >= r.n) return;
-  const uint64_t index = static_cast<uint64_t>(s) * static_cast<uint64_t>(r.n) + static_cast<uint64_t>(v);
+  const uint64_t index = static_cast<uint64_t>(it.slot) * static_cast<uint64_t>(r.n) + static_cast<uint64_t>(v);
   r.pop.labels[index] = static_cast<uint8_t>(InitialLabel(r.init_stream, index, r.k));
 }
 
-__global__ void __launch_bounds__(kBlock) KernelDegrees(RunDev r) {
+__global__ void __launch_bounds__(kBlock) KernelDegrees(const RunDev* run) {
+  const RunDev& r = *run;
   const int v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= r.n) return;
   const uint64_t* row = r.bits + static_cast<size_t>(v) * r.words;
   int acc = 0;
   for (int w = 0; w < r.words; ++w) acc += __popcll(row[w]);
-  r.deg[v] = acc;  // deg is written only here
+  r.deg[v] = acc;
 }
 
 // Rebuild, part 1: per slot, the cluster masks (one ballot per cluster per 64 labels) and the cluster sizes.
-__global__ void __launch_bounds__(kBlock) KernelMasks(RunDev r) {
+__global__ void __launch_bounds__(kBlock) KernelMasks(const RunDev* runs, const Item* items) {
+  const Item it = items[blockIdx.x];
+  const RunDev& r = runs[it.run];
   if (r.ctl->stopped) return;
-  const int s = blockIdx.x;
+  const int s = it.slot;
   const int n = r.n, k = r.k, words = r.words;
   const int lane = threadIdx.x & 63;
   const uint8_t* labels = r.pop.labels + static_cast<size_t>(s) * n;
@@ -69,20 +80,23 @@ __global__ void __launch_bounds__(kBlock) KernelMasks(RunDev r) {
   if (threadIdx.x == 0) r.intra[s] = 0;
 }
 
-// Rebuild, part 2: G[s][c][v] = sum_w popc(row_v[w] & mask_{s,c}[w]) for a tile of kRows vertices x kSlots slots x
-// kClusters clusters. The masks of the tile are staged through LDS in chunks of kWordTile words; every thread owns
-// one row. The diagonal G[s][label v][v] is summed into intra[s] for the objective.
+// Rebuild, part 2: G[s][c][v] = sum_w popc(row_v[w] & mask_{s,c}[w]) for a tile of kRebuildRows vertices x two
+// slots of one run x kClusters clusters. The masks of the tile are staged through LDS in chunks of kWordTile words;
+// every thread owns one row. The diagonal G[s][label v][v] is summed into intra[s] for the objective.
 constexpr int kRebuildRows = 256;
 constexpr int kRebuildSlots = 2;
 constexpr int kWordTile = 256;
 
 template <typename GT, int kClusters>
-__global__ void __launch_bounds__(kRebuildRows) KernelRebuildG(RunDev r) {
+__global__ void __launch_bounds__(kRebuildRows) KernelRebuildG(const RunDev* runs, const Item* pairs) {
+  const Item pair = pairs[blockIdx.y];
+  const RunDev& r = runs[pair.run];
   if (r.ctl->stopped) return;
   const int n = r.n, k = r.k, words = r.words, population = r.population;
-  const int v = blockIdx.x * kRebuildRows + threadIdx.x;
-  const int s0 = blockIdx.y * kRebuildSlots;
   const int c0 = blockIdx.z * kClusters;
+  if (c0 >= k || static_cast<int>(blockIdx.x) * kRebuildRows >= n) return;  // uniform per block
+  const int v = blockIdx.x * kRebuildRows + threadIdx.x;
+  const int s0 = pair.slot;
   __shared__ uint64_t tile[kRebuildSlots][kClusters][kWordTile];
   __shared__ int64_t block_intra[kRebuildSlots];
   if (threadIdx.x < kRebuildSlots) block_intra[threadIdx.x] = 0;
@@ -143,12 +157,14 @@ __global__ void __launch_bounds__(kRebuildRows) KernelRebuildG(RunDev r) {
 }
 
 // Rebuild, part 3: f = m + sum_c C(s_c, 2) - 2W with 2W = intra (reference include/cc_math.hpp:66-68).
-__global__ void __launch_bounds__(kBlock) KernelFinalize(RunDev r) {
+__global__ void __launch_bounds__(kBlock) KernelFinalize(const RunDev* runs, const Item* items, int count) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const Item it = items[i];
+  const RunDev& r = runs[it.run];
   if (r.ctl->stopped) return;
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= r.population) return;
-  const int32_t* sizes = r.pop.sizes + static_cast<size_t>(s) * r.k;
-  r.pop.f[s] = r.edges + SamePairs(sizes, r.k) - r.intra[s];
+  const int32_t* sizes = r.pop.sizes + static_cast<size_t>(it.slot) * r.k;
+  r.pop.f[it.slot] = r.edges + SamePairs(sizes, r.k) - r.intra[it.slot];
 }
 
 // Best (f, slot) of a block: the first minimum.
@@ -182,7 +198,6 @@ __device__ FSlot BlockArgMin(FSlot x, FSlot* scratch /*[16]*/) {
 __device__ int64_t BlockSum(int64_t x, int64_t* scratch /*[16]*/) {
   for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
   const int wave = threadIdx.x >> 6;
-  __syncthreads();
   if ((threadIdx.x & 63) == 0) scratch[wave] = x;
   __syncthreads();
   int64_t total = 0;
@@ -194,11 +209,12 @@ __device__ int64_t BlockSum(int64_t x, int64_t* scratch /*[16]*/) {
 constexpr int kRecordThreads = 1024;
 constexpr int kRankLdsSlots = 4096;
 
-// The record after a phase: the first slot of minimal f, taken only if strictly better (reference
-// src/cc_pbils_cpu.cpp:208-214, cuda/cc_pbils_gpu.cu:324-334). In iteration mode it also does Progress::Next
-// (include/cc_pbils.hpp:69-78) and the GWW ranking by (f, slot) (src/cc_pbils_cpu.cpp:168-175, 234-239), leaving in
-// src[] the slot of `next` that each new slot copies.
-__global__ void __launch_bounds__(kRecordThreads) KernelRecord(RunDev r, int mode, int read_next, int32_t* order) {
+// The record after a phase, one block per run: the first slot of minimal f, taken only if strictly better
+// (reference src/cc_pbils_cpu.cpp:208-214, cuda/cc_pbils_gpu.cu:324-334). In iteration mode it also does
+// Progress::Next (include/cc_pbils.hpp:69-78) and the GWW ranking by (f, slot) (src/cc_pbils_cpu.cpp:168-175,
+// 234-239), leaving in src[] the slot of `next` that each new slot copies.
+__global__ void __launch_bounds__(kRecordThreads) KernelRecord(const RunDev* runs, int mode, int read_next) {
+  const RunDev& r = runs[blockIdx.x];
   Ctl* ctl = r.ctl;
   if (ctl->stopped) return;
   const PopDev& p = read_next ? r.next : r.pop;
@@ -266,10 +282,10 @@ __global__ void __launch_bounds__(kRecordThreads) KernelRecord(RunDev r, int mod
         const int64_t ft = f[t];
         rank += (ft < fs) | ((ft == fs) & (t < s));
       }
-      order[rank] = s;
+      r.order[rank] = s;
     }
     __syncthreads();
-    for (int i = tid; i < r.copies; i += kRecordThreads) r.src[order[population - 1 - i]] = order[i];
+    for (int i = tid; i < r.copies; i += kRecordThreads) r.src[r.order[population - 1 - i]] = r.order[i];
   }
   if (tid == 0) {
     ctl->perturb_iteration = ctl->iteration;
@@ -279,11 +295,13 @@ __global__ void __launch_bounds__(kRecordThreads) KernelRecord(RunDev r, int mod
 
 // The next population: slot s starts from next[src[s]] (GWW) and is perturbed with the per-label stream
 // (reference KernelPerturb and Perturb, src/cc_pbils_cpu.cpp:151-161: index = slot * n + v).
-__global__ void __launch_bounds__(kBlock) KernelPerturb(RunDev r) {
+__global__ void __launch_bounds__(kBlock) KernelPerturb(const RunDev* runs, const Item* items) {
+  const Item it = items[blockIdx.y];
+  const RunDev& r = runs[it.run];
   if (r.ctl->stopped) return;
   const int v = blockIdx.x * kBlock + threadIdx.x;
-  const int s = blockIdx.y;
   if (v >= r.n) return;
+  const int s = it.slot;
   const int from = r.src[s];
   int label = r.next.labels[static_cast<size_t>(from) * r.n + v];
   if (r.do_perturb) {
@@ -295,7 +313,8 @@ __global__ void __launch_bounds__(kBlock) KernelPerturb(RunDev r) {
 
 // Start partition of start vertex first + b: v and its neighbours -> 0, the rest -> 1 (reference
 // KernelNeighborhoodLabels, cuda/cc_population_kernels.cuh:15-22).
-__global__ void __launch_bounds__(kBlock) KernelNeighborhoodLabels(RunDev r, int first) {
+__global__ void __launch_bounds__(kBlock) KernelNeighborhoodLabels(const RunDev* run, int first) {
+  const RunDev& r = *run;
   const int u = blockIdx.x * kBlock + threadIdx.x;
   const int b = blockIdx.y;
   if (u >= r.n) return;
@@ -305,55 +324,70 @@ __global__ void __launch_bounds__(kBlock) KernelNeighborhoodLabels(RunDev r, int
   r.pop.labels[static_cast<size_t>(b) * r.n + u] = (u == v || joined) ? 0 : 1;
 }
 
+int Blocks(int count, int per) { return (count + per - 1) / per; }
+
 }  // namespace
 
-void LaunchResetCtl(const RunDev& r, hipStream_t stream) { KernelResetCtl<<<1, 64, 0, stream>>>(r.ctl); }
-
-void LaunchInitLabels(const RunDev& r, hipStream_t stream) {
-  const dim3 grid((r.n + kBlock - 1) / kBlock, r.population);
-  KernelInitLabels<<<grid, kBlock, 0, stream>>>(r);
+void LaunchResetCtl(const RunDev* runs, int nruns, hipStream_t stream) {
+  KernelResetCtl<<<Blocks(nruns, kBlock), kBlock, 0, stream>>>(runs, nruns);
 }
 
-void LaunchDegrees(const RunDev& r, hipStream_t stream) {
-  KernelDegrees<<<(r.n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(r);
+void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream) {
+  if (count > 0) KernelResetQueues<<<Blocks(count, kBlock), kBlock, 0, stream>>>(queues, count);
+}
+
+void LaunchDegrees(const RunDev* run, int n, hipStream_t stream) {
+  KernelDegrees<<<Blocks(n, kBlock), kBlock, 0, stream>>>(run);
+}
+
+void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream) {
+  if (items.count == 0) return;
+  KernelInitLabels<<<dim3(Blocks(max_n, kBlock), items.count), kBlock, 0, stream>>>(runs, items.items);
+}
+
+void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream) {
+  if (items.count == 0) return;
+  KernelMasks<<<items.count, kBlock, 0, stream>>>(runs, items.items);
 }
 
 template <typename GT>
-static void LaunchRebuildG(const RunDev& r, hipStream_t stream) {
-  const int row_blocks = (r.n + kRebuildRows - 1) / kRebuildRows;
-  const int slot_blocks = (r.population + kRebuildSlots - 1) / kRebuildSlots;
-  if (r.k <= 2) {
-    KernelRebuildG<GT, 2><<<dim3(row_blocks, slot_blocks, 1), kRebuildRows, 0, stream>>>(r);
-  } else if (r.k <= 4) {
-    KernelRebuildG<GT, 4><<<dim3(row_blocks, slot_blocks, 1), kRebuildRows, 0, stream>>>(r);
+static void LaunchRebuildGT(const RunDev* runs, const RebuildGroup& g, int max_n, hipStream_t stream) {
+  const dim3 grid(Blocks(max_n, kRebuildRows), g.pairs.count, 1);
+  if (g.kc == 2) {
+    KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
+  } else if (g.kc == 4) {
+    KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
   } else {
-    KernelRebuildG<GT, 8><<<dim3(row_blocks, slot_blocks, (r.k + 7) / 8), kRebuildRows, 0, stream>>>(r);
+    const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
+    KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
   }
 }
 
-void LaunchRebuild(const RunDev& r, int wide, hipStream_t stream) {
-  KernelMasks<<<r.population, kBlock, 0, stream>>>(r);
-  if (wide) {
-    LaunchRebuildG<int32_t>(r, stream);
+void LaunchRebuildG(const RunDev* runs, const RebuildGroup& group, int max_n, hipStream_t stream) {
+  if (group.pairs.count == 0) return;
+  if (group.wide) {
+    LaunchRebuildGT<int32_t>(runs, group, max_n, stream);
   } else {
-    LaunchRebuildG<int16_t>(r, stream);
+    LaunchRebuildGT<int16_t>(runs, group, max_n, stream);
   }
-  KernelFinalize<<<(r.population + kBlock - 1) / kBlock, kBlock, 0, stream>>>(r);
 }
 
-// `order` is a [population] int32 scratch for the GWW ranking.
-void LaunchRecord(const RunDev& r, int mode, int read_next, int32_t* order, hipStream_t stream) {
-  KernelRecord<<<1, kRecordThreads, 0, stream>>>(r, mode, read_next, order);
+void LaunchFinalize(const RunDev* runs, const ItemList& items, hipStream_t stream) {
+  if (items.count == 0) return;
+  KernelFinalize<<<Blocks(items.count, kBlock), kBlock, 0, stream>>>(runs, items.items, items.count);
 }
 
-void LaunchPerturb(const RunDev& r, hipStream_t stream) {
-  const dim3 grid((r.n + kBlock - 1) / kBlock, r.population);
-  KernelPerturb<<<grid, kBlock, 0, stream>>>(r);
+void LaunchRecord(const RunDev* runs, int nruns, int mode, int read_next, hipStream_t stream) {
+  KernelRecord<<<nruns, kRecordThreads, 0, stream>>>(runs, mode, read_next);
 }
 
-void LaunchNeighborhoodLabels(const RunDev& r, int first, int count, hipStream_t stream) {
-  const dim3 grid((r.n + kBlock - 1) / kBlock, count);
-  KernelNeighborhoodLabels<<<grid, kBlock, 0, stream>>>(r, first);
+void LaunchPerturb(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream) {
+  if (items.count == 0) return;
+  KernelPerturb<<<dim3(Blocks(max_n, kBlock), items.count), kBlock, 0, stream>>>(runs, items.items);
+}
+
+void LaunchNeighborhoodLabels(const RunDev* run, int n, int first, int count, hipStream_t stream) {
+  KernelNeighborhoodLabels<<<dim3(Blocks(n, kBlock), count), kBlock, 0, stream>>>(run, first);
 }
 
 }  // namespace cc::hip

@@ -1,10 +1,11 @@
-// cc_runtime.hip — host side of the HIP path: device graph, run memory, descent plan, the PBILS iteration loop,
-// Neighborhood batching, and the C ABI of include/cc_hip.h.
+// cc_runtime.hip — host side of the HIP path: device graphs, the descent plan, the batch engine (any number of
+// independent PBILS runs of different n and k advanced together), the solves built on it, and the C ABI of
+// include/cc_hip.h.
 //
 // Orchestration follows the reference's phase order (src/cc_pbils_cpu.cpp:183-252; cuda/cc_pbils_gpu.cu:347-387):
 // init -> rebuild -> record; per iteration: select + descent -> record -> GWW -> perturb -> rebuild. The record,
-// early-stop test and GWW ranking run on the device (KernelRecord), so iterations are queued in chunks and the host
-// synchronises once per chunk instead of twice per iteration.
+// early-stop test and GWW ranking run on the device (KernelRecord), so an iteration is a fixed sequence of launches
+// that is captured once in a HIP graph and replayed; the host synchronises once per chunk of iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,30 +13,19 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "cc_hip.h"
 #include "cc_internal.hpp"
+#include "cc_runtime.hpp"
 
 namespace cc::hip {
-namespace {
 
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
-};
-
-#define CC_HIP_CHECK(call)                                                                                     \
-  do {                                                                                                         \
-    const hipError_t status_ = (call);                                                                         \
-    if (status_ != hipSuccess) {                                                                               \
-      throw ::cc::hip::Error(CC_HIP_EDEVICE, std::string("hip error at ") + __FILE__ + ":" +                \
-                             std::to_string(__LINE__) +                                                       \
-                                      ": " + hipGetErrorString(status_));                                      \
-    }                                                                                                          \
-  } while (0)
+Error::Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
 
 void Fail(int code, const char* fmt, ...) {
   char text[512];
@@ -48,46 +38,27 @@ void Fail(int code, const char* fmt, ...) {
 
 std::string Mebibytes(size_t bytes) { return std::to_string((bytes + (1 << 20) - 1) >> 20) + " MiB"; }
 
-// Device allocations of one call, released together.
-class DeviceMemory {
- public:
-  DeviceMemory() = default;
-  DeviceMemory(const DeviceMemory&) = delete;
-  DeviceMemory& operator=(const DeviceMemory&) = delete;
-  ~DeviceMemory() {
-    for (void* p : blocks_) (void)hipFree(p);
-  }
-  template <typename T>
-  T* Allocate(size_t count) {
-    void* p = nullptr;
-    if (count == 0) return nullptr;
-    CC_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
-    blocks_.push_back(p);
-    return static_cast<T*>(p);
-  }
+DeviceMemory::~DeviceMemory() {
+  for (void* p : blocks_) (void)hipFree(p);
+}
 
- private:
-  std::vector<void*> blocks_;
-};
-
-}  // namespace
-
-// The graph on the device: the bitset and the degrees, uploaded once.
-struct DeviceGraph {
-  uint32_t n = 0;
-  uint32_t words = 0;
-  uint64_t edges = 0;
-  int device = 0;
-  uint64_t* bits = nullptr;
-  int32_t* deg = nullptr;
-  DeviceMemory memory;
-};
-
-namespace {
+void* DeviceMemory::AllocateBytes(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  void* p = nullptr;
+  CC_HIP_CHECK(hipMalloc(&p, bytes));
+  blocks_.push_back(p);
+  return p;
+}
 
 uint32_t Words64(uint32_t n) { return std::max<uint32_t>(1, (n + 63) / 64); }
 
-void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges) {
+size_t FreeDeviceBytes() {
+  size_t free_bytes = 0, total_bytes = 0;
+  CC_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
+  return free_bytes;
+}
+
+void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream) {
   if (n == 0) Fail(CC_HIP_EINVAL, "graph has no vertices");
   if (n > static_cast<uint32_t>(kMaxVerticesWide)) {
     Fail(CC_HIP_EINVAL, "n exceeds %d, the range of the packed move", kMaxVerticesWide);
@@ -97,23 +68,25 @@ void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges) {
   g.words = Words64(n);
   g.edges = edges;
   const size_t count = static_cast<size_t>(n) * g.words;
-  size_t free_bytes = 0, total_bytes = 0;
-  CC_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
-  if (count * 8 + n * 4 > free_bytes) {
-    Fail(CC_HIP_ENOMEM, "graph needs %s of device memory, %s free", Mebibytes(count * 8).c_str(),
+  const size_t bytes = count * 8 + static_cast<size_t>(n) * 4 + sizeof(RunDev);
+  const size_t free_bytes = FreeDeviceBytes();
+  if (bytes > free_bytes) {
+    Fail(CC_HIP_ENOMEM, "graph needs %s of device memory, %s free", Mebibytes(bytes).c_str(),
          Mebibytes(free_bytes).c_str());
   }
-  g.bits = g.memory.Allocate<uint64_t>(count);
-  g.deg = g.memory.Allocate<int32_t>(n);
-  CC_HIP_CHECK(hipMemcpy(g.bits, bits, count * sizeof(uint64_t), hipMemcpyHostToDevice));
+  g.bits = static_cast<uint64_t*>(g.memory.AllocateBytes(count * sizeof(uint64_t)));
+  g.deg = static_cast<int32_t*>(g.memory.AllocateBytes(static_cast<size_t>(n) * sizeof(int32_t)));
+  RunDev* probe = static_cast<RunDev*>(g.memory.AllocateBytes(sizeof(RunDev)));
+  CC_HIP_CHECK(hipMemcpyAsync(g.bits, bits, count * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
   RunDev r{};
   r.bits = g.bits;
   r.deg = g.deg;
   r.n = static_cast<int32_t>(n);
   r.words = static_cast<int32_t>(g.words);
-  LaunchDegrees(r, nullptr);
+  CC_HIP_CHECK(hipMemcpyAsync(probe, &r, sizeof(RunDev), hipMemcpyHostToDevice, stream));
+  LaunchDegrees(probe, static_cast<int>(n), stream);
   CC_HIP_CHECK(hipGetLastError());
-  CC_HIP_CHECK(hipDeviceSynchronize());
+  CC_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // reference CheckParams (src/cc_pbils_cpu.cpp:177-181) plus the device limits of SolveGpu (cuda/cc_pbils_gpu.cu:441-454)
@@ -129,6 +102,8 @@ bool WidePath(uint32_t n, const cc_pbils_params& p) {
   return n > static_cast<uint32_t>(kMaxVerticesNarrow) || p.ls_kernel == CC_LS_WIDE;
 }
 
+namespace {
+
 int GenericThreads(int n) {
   for (int t : {256, 512}) {
     if (t * 8 >= n) return t;
@@ -136,12 +111,15 @@ int GenericThreads(int n) {
   return 1024;
 }
 
+}  // namespace
+
 // The descent kernel and its launch shape (reference PlanDescent, cuda/cc_pbils_gpu.cu:69-102).
-DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, int wide) {
+DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
   const int n = static_cast<int>(n_);
+  const int wide = WidePath(n_, p) ? 1 : 0;
   DescentPlan plan;
   plan.k = k;
-  plan.blocks = p.population;
+  plan.wide = wide;
   plan.threads = GenericThreads(n);
   const size_t lds = GenericLdsBytes(n, k, wide);
   const bool fits = lds <= DescentLdsLimit();
@@ -193,138 +171,323 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, int wide) 
   return plan;
 }
 
-using Clock = std::chrono::steady_clock;
-
 double Since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-// HIP event pairs around launches, summed after the chunk's synchronisation.
-class Timer {
+// ---- timing ------------------------------------------------------------------------------------------------------
+
+Timer::~Timer() {
+  for (auto& e : events_) (void)hipEventDestroy(e);
+}
+
+void Timer::Record(hipStream_t s) {
+  if (used_ == events_.size()) {
+    hipEvent_t e;
+    CC_HIP_CHECK(hipEventCreate(&e));
+    events_.push_back(e);
+  }
+  CC_HIP_CHECK(hipEventRecord(events_[used_++], s));
+}
+
+void Timer::End(hipStream_t s, double* total_ms, int64_t* launches) {
+  if (!on_) return;
+  Record(s);
+  pending_.push_back({used_ - 2, total_ms, launches});
+}
+
+void Timer::Collect() {
+  for (const auto& p : pending_) {
+    float ms = 0.f;
+    CC_HIP_CHECK(hipEventElapsedTime(&ms, events_[p.first], events_[p.first + 1]));
+    *p.total += ms;
+    *p.launches += 1;
+  }
+  pending_.clear();
+  used_ = 0;
+}
+
+Stream::Stream() { CC_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+Stream::~Stream() { (void)hipStreamDestroy(s_); }
+
+// ---- the batch engine ------------------------------------------------------------------------------------------
+
+namespace {
+
+// Two passes over the same layout: the first sums the bytes, the second hands out 256-byte-aligned pointers of one
+// allocation.
+class Arena {
  public:
-  explicit Timer(bool on) : on_(on) {}
-  ~Timer() {
-    for (auto& e : events_) (void)hipEventDestroy(e);
+  template <typename T>
+  void Take(T*& ptr, size_t count) {
+    const size_t bytes = (count * sizeof(T) + 255) & ~static_cast<size_t>(255);
+    ptr = base_ ? reinterpret_cast<T*>(base_ + used_) : nullptr;
+    if (count == 0) ptr = nullptr;
+    used_ += bytes;
   }
-  void Begin(hipStream_t s) {
-    if (on_) Record(s);
-  }
-  void End(hipStream_t s, double* total_ms, int64_t* launches) {
-    if (!on_) return;
-    Record(s);
-    pending_.push_back({used_ - 2, total_ms, launches});
-  }
-  void Collect() {
-    for (const auto& p : pending_) {
-      float ms = 0.f;
-      CC_HIP_CHECK(hipEventElapsedTime(&ms, events_[p.first], events_[p.first + 1]));
-      *p.total += ms;
-      *p.launches += 1;
-    }
-    pending_.clear();
+  size_t used() const { return used_; }
+  void Start(uint8_t* base) {
+    base_ = base;
     used_ = 0;
   }
 
  private:
-  void Record(hipStream_t s) {
-    if (used_ == events_.size()) {
-      hipEvent_t e;
-      CC_HIP_CHECK(hipEventCreate(&e));
-      events_.push_back(e);
-    }
-    CC_HIP_CHECK(hipEventRecord(events_[used_++], s));
-  }
-  struct Pending {
-    size_t first;
-    double* total;
-    int64_t* launches;
-  };
-  bool on_;
-  std::vector<hipEvent_t> events_;
-  std::vector<Pending> pending_;
+  uint8_t* base_ = nullptr;
   size_t used_ = 0;
 };
 
-class Stream {
- public:
-  Stream() { CC_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
-  ~Stream() { (void)hipStreamDestroy(s_); }
-  hipStream_t get() const { return s_; }
+int KcOf(int k) { return k <= 2 ? 2 : (k <= 4 ? 4 : 8); }
 
- private:
-  hipStream_t s_ = nullptr;
-};
+// Expected relative cost of one descent of a run, to order the persistent queues largest first.
+double DescentCost(const RunSpec& s) {
+  const double n = s.graph->n;
+  return n * n * static_cast<double>(s.params.k);
+}
 
-// Device memory of one run (reference Run, cuda/cc_pbils_gpu.cu:146-246).
-struct RunMemory {
-  DeviceMemory memory;
-  RunDev r{};
-  int32_t* order = nullptr;
-  Ctl* host_ctl = nullptr;  // pinned
+}  // namespace
 
-  RunMemory(const DeviceGraph& g, const cc_pbils_params& p, int population, bool wide, bool has_next,
-            const DescentPlan& plan) {
-    const size_t n = g.n;
-    const size_t k = static_cast<size_t>(p.k);
-    const size_t P = static_cast<size_t>(population);
-    const size_t gsize = wide ? 4 : 2;
-    const size_t iterations = static_cast<size_t>(std::max(p.iterations, 1));
-    const size_t pop_bytes = P * n + P * k * n * gsize + P * k * 4 + P * 8;
-    const size_t work_words = plan.descent == Descent::kK3Wide ? P * n : 0;
-    const size_t bytes = pop_bytes * (has_next ? 2 : 1) + P * 8 * 2 + P * 4 * 2 + n + iterations * 8 +
-                         P * k * g.words * 8 + work_words * 8 + sizeof(Ctl) + 4096;
-    size_t free_bytes = 0, total_bytes = 0;
-    CC_HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
-    if (bytes > free_bytes) {
-      Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes).c_str(),
-           Mebibytes(free_bytes).c_str());
+Batch::Batch(std::vector<RunSpec> specs, hipStream_t stream) : specs_(std::move(specs)), stream_(stream) {
+  const int nruns = static_cast<int>(specs_.size());
+  if (nruns == 0) Fail(CC_HIP_EINVAL, "empty batch");
+  host_runs_.resize(nruns);
+  for (const RunSpec& s : specs_) max_n_ = std::max(max_n_, static_cast<int>(s.graph->n));
+
+  // item tables (host)
+  std::vector<Item> all;
+  std::map<std::pair<int, int>, std::vector<Item>> pairs;  // (wide, kc) -> pairs
+  std::vector<std::pair<DescentPlan, std::vector<Item>>> groups;
+  for (int i = 0; i < nruns; ++i) {
+    const RunSpec& s = specs_[i];
+    for (int slot = 0; slot < s.population; ++slot) all.push_back({i, slot});
+    auto& pl = pairs[{s.plan.wide, KcOf(s.params.k)}];
+    for (int slot = 0; slot < s.population; slot += 2) pl.push_back({i, slot});
+    size_t g = 0;
+    while (g < groups.size() && !SameLaunch(groups[g].first, s.plan)) ++g;
+    if (g == groups.size()) groups.push_back({s.plan, {}});
+    groups[g].first.lds_bytes = std::max(groups[g].first.lds_bytes, s.plan.lds_bytes);
+    for (int slot = 0; slot < s.population; ++slot) groups[g].second.push_back({i, slot});
+  }
+  for (auto& g : groups) {
+    std::stable_sort(g.second.begin(), g.second.end(), [&](const Item& a, const Item& b) {
+      return DescentCost(specs_[a.run]) > DescentCost(specs_[b.run]);
+    });
+  }
+
+  // layout: per run buffers, then the item tables and queues
+  auto layout = [&](Arena& a) {
+    for (int i = 0; i < nruns; ++i) {
+      const RunSpec& s = specs_[i];
+      RunDev& r = host_runs_[i];
+      const size_t n = s.graph->n, k = static_cast<size_t>(s.params.k), P = static_cast<size_t>(s.population);
+      const size_t iterations = static_cast<size_t>(std::max(s.params.iterations, 1));
+      auto pop = [&](PopDev& d) {
+        a.Take(d.labels, P * n);
+        if (s.plan.wide) {
+          int32_t* g = nullptr;
+          a.Take(g, P * k * n);
+          d.g = g;
+        } else {
+          int16_t* g = nullptr;
+          a.Take(g, P * k * n);
+          d.g = g;
+        }
+        a.Take(d.sizes, P * k);
+        a.Take(d.f, P);
+      };
+      pop(r.pop);
+      if (s.pbils) {
+        pop(r.next);
+      } else {
+        r.next = r.pop;
+      }
+      a.Take(r.moves, P);
+      a.Take(r.ctl, 1);
+      a.Take(r.rec_labels, n);
+      a.Take(r.trace, iterations);
+      a.Take(r.src, P);
+      a.Take(r.order, P);
+      a.Take(r.masks, P * k * s.graph->words);
+      a.Take(r.intra, P);
+      a.Take(r.work, s.plan.descent == Descent::kK3Wide ? P * n : 0);
     }
-    auto alloc_pop = [&](PopDev& d) {
-      d.labels = memory.Allocate<uint8_t>(P * n);
-      d.g = wide ? static_cast<void*>(memory.Allocate<int32_t>(P * k * n))
-                 : static_cast<void*>(memory.Allocate<int16_t>(P * k * n));
-      d.sizes = memory.Allocate<int32_t>(P * k);
-      d.f = memory.Allocate<int64_t>(P);
-    };
-    alloc_pop(r.pop);
-    if (has_next) {
-      alloc_pop(r.next);
-    } else {
-      r.next = r.pop;
+    a.Take(runs_, nruns);
+    a.Take(all_.items, all.size());
+    for (auto& [key, list] : pairs) {
+      RebuildGroup g{};
+      a.Take(const_cast<Item*&>(g.pairs.items), list.size());
+      rebuild_.push_back(g);
     }
-    r.bits = g.bits;
-    r.deg = g.deg;
-    r.n = static_cast<int32_t>(g.n);
-    r.words = static_cast<int32_t>(g.words);
+    for (auto& g : groups) {
+      ItemList l{};
+      a.Take(const_cast<Item*&>(l.items), g.second.size());
+      descent_.push_back({g.first, l});
+    }
+    a.Take(queues_, std::max<size_t>(groups.size(), 1));
+  };
+  Arena sizing;
+  layout(sizing);
+  rebuild_.clear();
+  descent_.clear();
+  bytes_ = sizing.used();
+  const size_t free_bytes = FreeDeviceBytes();
+  if (bytes_ > free_bytes) {
+    Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes_).c_str(),
+         Mebibytes(free_bytes).c_str());
+  }
+  uint8_t* base = static_cast<uint8_t*>(memory_.AllocateBytes(bytes_));
+  // Start from zeros: e.g. a Neighborhood batch without local search sums a moves[] that no descent writes.
+  CC_HIP_CHECK(hipMemsetAsync(base, 0, bytes_, stream_));
+  Arena arena;
+  arena.Start(base);
+  layout(arena);
+
+  // run descriptors
+  for (int i = 0; i < nruns; ++i) {
+    const RunSpec& s = specs_[i];
+    RunDev& r = host_runs_[i];
+    const cc_pbils_params& p = s.params;
+    r.bits = s.graph->bits;
+    r.deg = s.graph->deg;
+    r.n = static_cast<int32_t>(s.graph->n);
+    r.words = static_cast<int32_t>(s.graph->words);
     r.k = p.k;
-    r.population = population;
-    r.edges = static_cast<int64_t>(g.edges);
-    r.moves = memory.Allocate<int64_t>(P);
-    r.ctl = memory.Allocate<Ctl>(1);
-    r.rec_labels = memory.Allocate<uint8_t>(n);
-    r.trace = memory.Allocate<int64_t>(iterations);
-    r.src = memory.Allocate<int32_t>(P);
-    order = memory.Allocate<int32_t>(P);
-    r.masks = memory.Allocate<uint64_t>(P * k * g.words);
-    r.intra = memory.Allocate<int64_t>(P);
-    r.work = memory.Allocate<uint64_t>(work_words);
-    CC_HIP_CHECK(hipMemset(r.moves, 0, P * sizeof(int64_t)));
-    CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(Ctl), hipHostMallocDefault));
+    r.population = s.population;
+    r.edges = static_cast<int64_t>(s.graph->edges);
+    r.init_stream = StreamSeed(p.seed, kInitStream, 0);
+    r.select_stream = StreamSeed(p.seed, kSelectStream, 0);
+    r.perturb_stream = StreamSeed(p.seed, kPerturbStream, 0);
+    r.tournament = p.tournament;
+    r.copies = s.pbils ? static_cast<int32_t>(p.gww * p.population) : 0;  // WinnerCopies, src/cc_pbils_cpu.cpp:168
+    r.early_stop = p.early_stop;
+    r.iterations = p.iterations;
+    r.perturbation = static_cast<float>(p.perturbation);
+    r.do_perturb = (p.k >= 2 && p.perturbation > 0.0) ? 1 : 0;
+    r.select = s.pbils ? 1 : 0;
+    r.wide = s.plan.wide;
   }
-  ~RunMemory() {
-    if (host_ctl) (void)hipHostFree(host_ctl);
+  CC_HIP_CHECK(hipMemcpyAsync(runs_, host_runs_.data(), sizeof(RunDev) * nruns, hipMemcpyHostToDevice, stream_));
+  all_.count = static_cast<int32_t>(all.size());
+  CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(all_.items), all.data(), sizeof(Item) * all.size(),
+                              hipMemcpyHostToDevice, stream_));
+  size_t gi = 0;
+  for (auto& [key, list] : pairs) {
+    RebuildGroup& g = rebuild_[gi++];
+    g.wide = key.first;
+    g.kc = key.second;
+    g.pairs.count = static_cast<int32_t>(list.size());
+    for (const Item& it : list) g.max_k = std::max(g.max_k, specs_[it.run].params.k);
+    CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(g.pairs.items), list.data(), sizeof(Item) * list.size(),
+                                hipMemcpyHostToDevice, stream_));
   }
+  int cus = 0;
+  CC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, specs_[0].graph->device));
+  for (size_t g = 0; g < groups.size(); ++g) {
+    DescentGroup& d = descent_[g];
+    d.list.count = static_cast<int32_t>(groups[g].second.size());
+    d.list.queue = queues_ + g;
+    // Persistent workgroups: enough to occupy every CU; a surplus finds the queue empty and exits at once.
+    const int per_cu = d.plan.threads >= 1024 ? 1 : (d.plan.threads >= 512 ? 2 : 4);
+    d.plan.blocks = std::min(d.list.count, cus * per_cu);
+    if (!DescentSupported(d.plan)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
+    CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(d.list.items), groups[g].second.data(),
+                                sizeof(Item) * groups[g].second.size(), hipMemcpyHostToDevice, stream_));
+  }
+  CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), sizeof(Ctl) * nruns, hipHostMallocDefault));
+  CC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
 
-  void PullCtl(hipStream_t s) {
-    CC_HIP_CHECK(hipMemcpyAsync(host_ctl, r.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-    CC_HIP_CHECK(hipStreamSynchronize(s));
-  }
+Batch::~Batch() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (host_ctl_) (void)hipHostFree(host_ctl_);
+}
 
-  void PullLabels(int32_t* labels_out, hipStream_t s) {
-    std::vector<uint8_t> labels(r.n);
-    CC_HIP_CHECK(hipMemcpyAsync(labels.data(), r.rec_labels, labels.size(), hipMemcpyDeviceToHost, s));
-    CC_HIP_CHECK(hipStreamSynchronize(s));
-    for (int v = 0; v < r.n; ++v) labels_out[v] = labels[v];
+void Batch::Rebuild(Timer* timer, cc_pbils_stats* stats) {
+  if (timer) timer->Begin(stream_);
+  LaunchMasks(runs_, all_, stream_);
+  for (const RebuildGroup& g : rebuild_) LaunchRebuildG(runs_, g, max_n_, stream_);
+  LaunchFinalize(runs_, all_, stream_);
+  if (timer) timer->End(stream_, &stats->rebuild_ms, &stats->rebuild_launches);
+}
+
+void Batch::Descend(Timer* timer, cc_pbils_stats* stats) {
+  LaunchResetQueues(queues_, static_cast<int>(descent_.size()), stream_);
+  if (timer) timer->Begin(stream_);
+  for (const DescentGroup& d : descent_) LaunchDescent(runs_, d.list, d.plan, stream_);
+  if (timer) timer->End(stream_, &stats->search_ms, &stats->search_launches);
+}
+
+void Batch::Start() {
+  LaunchResetCtl(runs_, static_cast<int>(specs_.size()), stream_);
+  LaunchInitLabels(runs_, all_, max_n_, stream_);
+  Rebuild(nullptr, nullptr);
+  LaunchRecord(runs_, static_cast<int>(specs_.size()), kRecordInitial, 0, stream_);
+  CC_HIP_CHECK(hipGetLastError());
+}
+
+void Batch::EnqueueIteration(Timer* timer, cc_pbils_stats* stats) {
+  Descend(timer, stats);
+  LaunchRecord(runs_, static_cast<int>(specs_.size()), kRecordIteration, 1, stream_);
+  LaunchPerturb(runs_, all_, max_n_, stream_);
+  Rebuild(timer, stats);
+}
+
+void Batch::Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats) {
+  if (!use_graph || timer) {
+    for (int i = 0; i < count; ++i) EnqueueIteration(timer, stats);
+    CC_HIP_CHECK(hipGetLastError());
+    return;
   }
-};
+  if (!graph_exec_) {
+    // one iteration as a graph: the LDS attributes are set before the capture
+    for (const DescentGroup& d : descent_) {
+      const ItemList none{d.list.items, 0, d.list.queue};
+      LaunchDescent(runs_, none, d.plan, stream_);
+    }
+    CC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    EnqueueIteration(nullptr, nullptr);
+    CC_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+    CC_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < count; ++i) CC_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
+}
+
+const Ctl* Batch::PullCtl() {
+  CC_HIP_CHECK(hipMemcpyAsync(host_ctl_, host_runs_[0].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+  for (size_t i = 1; i < host_runs_.size(); ++i) {
+    CC_HIP_CHECK(hipMemcpyAsync(host_ctl_ + i, host_runs_[i].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+  }
+  CC_HIP_CHECK(hipStreamSynchronize(stream_));
+  return host_ctl_;
+}
+
+void Batch::PullLabels(int run, int32_t* labels_out) {
+  const RunDev& r = host_runs_[run];
+  std::vector<uint8_t> labels(r.n);
+  CC_HIP_CHECK(hipMemcpyAsync(labels.data(), r.rec_labels, labels.size(), hipMemcpyDeviceToHost, stream_));
+  CC_HIP_CHECK(hipStreamSynchronize(stream_));
+  for (int v = 0; v < r.n; ++v) labels_out[v] = labels[v];
+}
+
+void Batch::PullTrace(int run, int count, int64_t* out) {
+  if (count <= 0 || !out) return;
+  CC_HIP_CHECK(hipMemcpyAsync(out, host_runs_[run].trace, sizeof(int64_t) * count, hipMemcpyDeviceToHost, stream_));
+  CC_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// Neighborhood uses one run whose population is the batch of starts; the last batch may be smaller.
+void Batch::SetPopulation(int run, int population) {
+  host_runs_[run].population = population;
+  CC_HIP_CHECK(hipMemcpyAsync(runs_ + run, &host_runs_[run], sizeof(RunDev), hipMemcpyHostToDevice, stream_));
+}
+
+ItemList Batch::Prefix(const ItemList& list, int count) const {
+  ItemList l = list;
+  l.count = std::min(l.count, count);
+  return l;
+}
+
+// ---- solves ------------------------------------------------------------------------------------------------------
+
+namespace {
 
 int ClustersUsed(const int32_t* labels, uint32_t n, int k) {
   std::vector<char> seen(static_cast<size_t>(std::max(k, 1)), 0);
@@ -334,14 +497,6 @@ int ClustersUsed(const int32_t* labels, uint32_t n, int k) {
   int used = 0;
   for (char c : seen) used += c;
   return used;
-}
-
-void FillPlan(const DescentPlan& plan, cc_pbils_stats* stats) {
-  stats->descent = static_cast<int32_t>(plan.descent);
-  stats->threads = plan.threads;
-  stats->blocks = plan.blocks;
-  stats->vertices_per_thread = plan.vpt;
-  stats->lds_bytes = static_cast<int64_t>(plan.lds_bytes);
 }
 
 const char* DescentName(Descent d) {
@@ -356,117 +511,99 @@ const char* DescentName(Descent d) {
   return "?";
 }
 
-void PrintPlan(const DescentPlan& plan, size_t run_bytes) {
+void PrintPlan(const DescentPlan& plan, size_t bytes) {
   std::printf("  local search: %s, %d threads per block, %d blocks", DescentName(plan.descent), plan.threads,
               plan.blocks);
-  if (plan.descent == Descent::kK2Registers) {
-    std::printf(", %d vertices per thread%s", plan.vpt, plan.rows_in_lds ? ", adjacency in LDS" : "");
-  }
-  if (plan.descent == Descent::kGenericLds) std::printf(", %zu bytes of LDS", plan.lds_bytes);
-  std::printf("\n");
-  (void)run_bytes;
+  if (plan.vpt) std::printf(", %d vertices per thread", plan.vpt);
+  if (plan.rows_in_lds) std::printf(", adjacency in LDS");
+  if (plan.lds_bytes) std::printf(", %zu bytes of LDS", plan.lds_bytes);
+  std::printf("\n  device memory: %s\n", Mebibytes(bytes).c_str());
 }
 
-// PBILS on a device graph (reference gpu::Solve, cuda/cc_pbils_gpu.cu:347-387).
-void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_out, cc_pbils_stats* stats,
-                int64_t* trace_out) {
-  const auto started = Clock::now();
-  CheckParams(p);
-  const int wide = WidePath(g.n, p) ? 1 : 0;
-  const DescentPlan plan = PlanDescent(g.n, p.k, p, wide);
-  if (!DescentSupported(plan, wide)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
-  Stream stream;
-  const hipStream_t s = stream.get();
-  RunMemory run(g, p, p.population, wide != 0, true, plan);
-  RunDev& r = run.r;
-  r.init_stream = StreamSeed(p.seed, kInitStream, 0);
-  r.select_stream = StreamSeed(p.seed, kSelectStream, 0);
-  r.perturb_stream = StreamSeed(p.seed, kPerturbStream, 0);
-  r.tournament = p.tournament;
-  r.copies = static_cast<int32_t>(p.gww * p.population);  // WinnerCopies, src/cc_pbils_cpu.cpp:168
-  r.early_stop = p.early_stop;
-  r.iterations = p.iterations;
-  r.perturbation = static_cast<float>(p.perturbation);
-  r.do_perturb = (p.k >= 2 && p.perturbation > 0.0) ? 1 : 0;
-  r.select = 1;
-  if (p.verbose) PrintPlan(plan, 0);
+void FillPlan(const DescentPlan& plan, cc_pbils_stats* stats) {
+  stats->descent = static_cast<int32_t>(plan.descent);
+  stats->threads = plan.threads;
+  stats->blocks = plan.blocks;
+  stats->vertices_per_thread = plan.vpt;
+  stats->lds_bytes = static_cast<int64_t>(plan.lds_bytes);
+}
 
-  Timer timer(p.profile != 0);
-  Timer iteration_timer(p.profile != 0);
-  int64_t iteration_spans = 0;
-  LaunchResetCtl(r, s);
-  LaunchInitLabels(r, s);
-  LaunchRebuild(r, wide, s);
-  LaunchRecord(r, kRecordInitial, 0, run.order, s);
-  CC_HIP_CHECK(hipGetLastError());
+void FillStats(const Ctl& c, const RunSpec& spec, const int32_t* labels, cc_pbils_stats* stats) {
+  stats->objective = c.rec_f;
+  stats->iterations_done = c.iterations_done;
+  stats->local_searches = c.local_searches;
+  stats->accepted_moves = c.accepted_moves;
+  stats->clusters_used = ClustersUsed(labels, spec.graph->n, spec.params.k);
+}
 
+}  // namespace
+
+// Advances every run of the batch until all have stopped; chunks of iterations between host synchronisations grow
+// from 4 to 32 (1 with a time limit, which is checked between chunks).
+void RunToCompletion(Batch& batch, const std::vector<RunSpec>& specs, bool verbose, bool use_graph, double time_limit,
+                     Timer* timer, cc_pbils_stats* stats, Clock::time_point started) {
+  int max_iterations = 0;
+  for (const RunSpec& s : specs) max_iterations = std::max(max_iterations, s.params.iterations);
   int launched = 0;
-  int chunk = p.time_limit_sec > 0.0 ? 1 : 4;
+  int chunk = time_limit > 0.0 ? 1 : 4;
   int printed = 0;
   int64_t last_record = -1;
   int stall = 0;
-  if (p.verbose) {
-    run.PullCtl(s);
-    last_record = run.host_ctl->rec_f;
-  }
-  iteration_timer.Begin(s);
-  while (launched < p.iterations) {
-    const int count = std::min(chunk, p.iterations - launched);
-    for (int i = 0; i < count; ++i) {
-      timer.Begin(s);
-      LaunchDescent(r, plan, wide, s);
-      timer.End(s, &stats->search_ms, &stats->search_launches);
-      LaunchRecord(r, kRecordIteration, 1, run.order, s);
-      LaunchPerturb(r, s);
-      timer.Begin(s);
-      LaunchRebuild(r, wide, s);
-      timer.End(s, &stats->rebuild_ms, &stats->rebuild_launches);
-    }
-    CC_HIP_CHECK(hipGetLastError());
+  if (verbose) last_record = batch.PullCtl()[0].rec_f;
+  while (launched < max_iterations) {
+    const int count = std::min(chunk, max_iterations - launched);
+    batch.Iterate(count, use_graph, timer, stats);
     launched += count;
-    run.PullCtl(s);
-    timer.Collect();
-    const Ctl& c = *run.host_ctl;
-    if (p.verbose && c.iterations_done > printed) {
-      std::vector<int64_t> trace(c.iterations_done);
-      CC_HIP_CHECK(hipMemcpy(trace.data(), r.trace, trace.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-      for (int it = printed; it < c.iterations_done; ++it) {
+    const Ctl* ctl = batch.PullCtl();
+    if (timer) timer->Collect();
+    if (verbose && ctl[0].iterations_done > printed) {
+      std::vector<int64_t> trace(ctl[0].iterations_done);
+      batch.PullTrace(0, ctl[0].iterations_done, trace.data());
+      for (int it = printed; it < ctl[0].iterations_done; ++it) {
         stall = trace[it] < last_record ? 0 : stall + 1;
         last_record = trace[it];
         std::printf("  iter %3d  record %lld  stall %d  %.3fs\n", it + 1, static_cast<long long>(trace[it]), stall,
                     Since(started));
       }
-      printed = c.iterations_done;
+      printed = ctl[0].iterations_done;
     }
-    if (c.stopped) break;
-    if (p.time_limit_sec > 0.0 && Since(started) >= p.time_limit_sec) break;
+    bool all_stopped = true;
+    for (size_t i = 0; i < specs.size(); ++i) all_stopped &= ctl[i].stopped != 0;
+    if (all_stopped) break;
+    if (time_limit > 0.0 && Since(started) >= time_limit) break;
     chunk = std::min(chunk * 2, 32);
   }
-  iteration_timer.End(s, &stats->iteration_ms, &iteration_spans);
-  run.PullCtl(s);
-  iteration_timer.Collect();
-  const Ctl c = *run.host_ctl;
+}
+
+// PBILS on a device graph (reference gpu::Solve, cuda/cc_pbils_gpu.cu:347-387).
+void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_out, cc_pbils_stats* stats,
+                int64_t* trace_out, Clock::time_point started) {
+  CheckParams(p);
+  RunSpec spec{&g, p, PlanDescent(g.n, p.k, p), p.population, true};
+  Stream stream;
+  Batch batch({spec}, stream.get());
+  if (p.verbose) PrintPlan(batch.Plan(0), batch.Bytes());
+  Timer timer(p.profile != 0);
+  Timer span(p.profile != 0);
+  int64_t spans = 0;
+  batch.Start();
+  span.Begin(stream.get());
+  RunToCompletion(batch, {spec}, p.verbose != 0, true, p.time_limit_sec, p.profile ? &timer : nullptr, stats, started);
+  span.End(stream.get(), &stats->iteration_ms, &spans);
+  const Ctl c = batch.PullCtl()[0];
+  span.Collect();
   if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
-  run.PullLabels(labels_out, s);
-  if (trace_out && c.iterations_done > 0) {
-    CC_HIP_CHECK(hipMemcpy(trace_out, r.trace, static_cast<size_t>(c.iterations_done) * sizeof(int64_t),
-                           hipMemcpyDeviceToHost));
-  }
-  stats->objective = c.rec_f;
-  stats->iterations_done = c.iterations_done;
-  stats->local_searches = c.local_searches;
-  stats->accepted_moves = c.accepted_moves;
-  stats->clusters_used = ClustersUsed(labels_out, g.n, p.k);
-  FillPlan(plan, stats);
-  stats->seconds = Since(started);
+  batch.PullLabels(0, labels_out);
+  batch.PullTrace(0, c.iterations_done, trace_out);
+  FillStats(c, spec, labels_out, stats);
+  FillPlan(batch.Plan(0), stats);
 }
 
 // Neighborhood for k = 2 in batches of at most `population` start vertices (reference SolveNeighborhood,
 // cuda/cc_pbils_gpu.cu:389-436). Starts are processed in vertex order and the record only moves on a strictly
 // smaller f, so ties keep the smaller start vertex across batches.
 void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool local_search, int32_t* labels_out,
-                       cc_pbils_stats* stats) {
-  const auto started = Clock::now();
+                       cc_pbils_stats* stats, Clock::time_point started) {
   if (params.k != 2) Fail(CC_HIP_EINVAL, "Neighborhood algorithms require k = 2");
   if (params.population < 1) Fail(CC_HIP_EINVAL, "population must be >= 1");
   if (params.ls_kernel < CC_LS_AUTO || params.ls_kernel > CC_LS_WIDE) {
@@ -477,52 +614,77 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   p.population = std::min(params.population, n);
   p.gww = 0.0;
   p.iterations = 1;
-  const int wide = WidePath(g.n, p) ? 1 : 0;
-  DescentPlan plan = PlanDescent(g.n, 2, p, wide);
-  if (!DescentSupported(plan, wide)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
+  p.early_stop = 1;
+  RunSpec spec{&g, p, PlanDescent(g.n, 2, p), p.population, false};
   Stream stream;
   const hipStream_t s = stream.get();
-  RunMemory run(g, p, p.population, wide != 0, false, plan);
-  RunDev& r = run.r;
-  r.select = 0;
-  r.iterations = 1;
-  r.early_stop = 1;
-  if (p.verbose) PrintPlan(plan, 0);
+  Batch batch({spec}, s);
+  if (p.verbose) PrintPlan(batch.Plan(0), batch.Bytes());
   Timer timer(p.profile != 0);
-  LaunchResetCtl(r, s);
+  Timer* t = p.profile ? &timer : nullptr;
+  LaunchResetCtl(batch.Runs(), 1, s);
   const int capacity = p.population;
   for (int first = 0; first < n; first += capacity) {
-    RunDev b = r;
-    b.population = std::min(capacity, n - first);
-    LaunchNeighborhoodLabels(b, first, b.population, s);
-    timer.Begin(s);
-    LaunchRebuild(b, wide, s);
-    timer.End(s, &stats->rebuild_ms, &stats->rebuild_launches);
-    if (local_search) {
-      timer.Begin(s);
-      LaunchDescent(b, plan, wide, s);
-      timer.End(s, &stats->search_ms, &stats->search_launches);
+    const int count = std::min(capacity, n - first);
+    if (count != capacity) batch.SetPopulation(0, count);
+    LaunchNeighborhoodLabels(batch.Runs(), n, first, count, s);
+    if (t) t->Begin(s);
+    LaunchMasks(batch.Runs(), batch.Prefix(batch.AllItems(), count), s);
+    for (const RebuildGroup& rg : batch.RebuildGroups()) {
+      RebuildGroup prefix = rg;
+      prefix.pairs = batch.Prefix(rg.pairs, (count + 1) / 2);
+      LaunchRebuildG(batch.Runs(), prefix, n, s);
     }
-    LaunchRecord(b, kRecordBatch, 1, run.order, s);
+    LaunchFinalize(batch.Runs(), batch.Prefix(batch.AllItems(), count), s);
+    if (t) t->End(s, &stats->rebuild_ms, &stats->rebuild_launches);
+    if (local_search) {
+      const DescentGroup& d = batch.DescentGroups()[0];
+      DescentPlan plan = d.plan;
+      plan.blocks = std::min(plan.blocks, count);
+      LaunchResetQueues(d.list.queue, 1, s);
+      if (t) t->Begin(s);
+      LaunchDescent(batch.Runs(), batch.Prefix(d.list, count), plan, s);
+      if (t) t->End(s, &stats->search_ms, &stats->search_launches);
+    }
+    LaunchRecord(batch.Runs(), 1, kRecordBatch, 1, s);
     CC_HIP_CHECK(hipGetLastError());
     if (p.verbose) {
-      run.PullCtl(s);
-      std::printf("  neighborhoods %d/%d  record %lld  %.3fs\n", first + b.population, n,
-                  static_cast<long long>(run.host_ctl->rec_f), Since(started));
+      std::printf("  neighborhoods %d/%d  record %lld  %.3fs\n", first + count, n,
+                  static_cast<long long>(batch.PullCtl()[0].rec_f), Since(started));
     }
   }
-  run.PullCtl(s);
-  timer.Collect();
-  const Ctl c = *run.host_ctl;
+  const Ctl c = batch.PullCtl()[0];
+  if (t) t->Collect();
   if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
-  run.PullLabels(labels_out, s);
-  stats->objective = c.rec_f;
+  batch.PullLabels(0, labels_out);
+  FillStats(c, spec, labels_out, stats);
   stats->iterations_done = 0;
   stats->local_searches = local_search ? n : 0;
-  stats->accepted_moves = c.accepted_moves;
-  stats->clusters_used = ClustersUsed(labels_out, g.n, 2);
-  FillPlan(plan, stats);
-  stats->seconds = Since(started);
+  FillPlan(batch.Plan(0), stats);
+}
+
+// Many independent PBILS runs advanced together: one launch per phase for all runs, descents grouped by kernel
+// instance and claimed from persistent work queues, iterations replayed from one HIP graph.
+void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
+  std::vector<RunSpec> specs;
+  for (BatchRun& b : runs) {
+    CheckParams(b.params);
+    specs.push_back({b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params), b.params.population, true});
+  }
+  Stream stream;
+  Batch batch(specs, stream.get());
+  batch.Start();
+  cc_pbils_stats unused{};
+  RunToCompletion(batch, specs, false, use_graph, 0.0, nullptr, &unused, Clock::now());
+  const Ctl* ctl = batch.PullCtl();
+  std::vector<Ctl> copy(ctl, ctl + specs.size());
+  for (size_t i = 0; i < runs.size(); ++i) {
+    if (copy[i].error) Fail(CC_HIP_EDEVICE, "run %zu: a descent exceeded its move cap", i);
+    batch.PullLabels(static_cast<int>(i), runs[i].labels);
+    batch.PullTrace(static_cast<int>(i), copy[i].iterations_done, runs[i].trace);
+    FillStats(copy[i], specs[i], runs[i].labels, runs[i].stats);
+    FillPlan(batch.Plan(static_cast<int>(i)), runs[i].stats);
+  }
 }
 
 int Report(const std::exception& e, char* err, size_t err_len, int code) {
@@ -533,26 +695,7 @@ int Report(const std::exception& e, char* err, size_t err_len, int code) {
   return code;
 }
 
-template <typename Body>
-int Guard(char* err, size_t err_len, Body&& body) {
-  try {
-    body();
-    return CC_HIP_OK;
-  } catch (const Error& e) {
-    return Report(e, err, err_len, e.code);
-  } catch (const std::exception& e) {
-    return Report(e, err, err_len, CC_HIP_EDEVICE);
-  }
-}
-
-void ClearStats(cc_pbils_stats* stats) { std::memset(stats, 0, sizeof(*stats)); }
-
-}  // namespace
 }  // namespace cc::hip
-
-struct cc_hip_graph {
-  cc::hip::DeviceGraph g;
-};
 
 extern "C" {
 
@@ -577,14 +720,10 @@ int cc_hip_graph_upload(const uint64_t* bits, uint32_t n, uint64_t edges, cc_hip
                         size_t err_len) {
   *out = nullptr;
   return cc::hip::Guard(err, err_len, [&] {
-    auto* h = new cc_hip_graph();
-    try {
-      cc::hip::Upload(h->g, bits, n, edges);
-    } catch (...) {
-      delete h;
-      throw;
-    }
-    *out = h;
+    auto h = std::make_unique<cc_hip_graph>();
+    cc::hip::Stream stream;
+    cc::hip::Upload(h->g, bits, n, edges, stream.get());
+    *out = h.release();
   });
 }
 
@@ -592,31 +731,38 @@ void cc_hip_graph_free(cc_hip_graph* graph) { delete graph; }
 
 int cc_hip_graph_solve_pbils(cc_hip_graph* graph, const cc_pbils_params* params, int32_t* labels_out,
                              cc_pbils_stats* stats, int64_t* trace_out, char* err, size_t err_len) {
-  cc::hip::ClearStats(stats);
+  std::memset(stats, 0, sizeof(*stats));
   return cc::hip::Guard(err, err_len, [&] {
+    const auto started = cc::hip::Clock::now();
     CC_HIP_CHECK(hipSetDevice(graph->g.device));
-    cc::hip::SolvePbils(graph->g, *params, labels_out, stats, trace_out);
+    cc::hip::SolvePbils(graph->g, *params, labels_out, stats, trace_out, started);
+    stats->seconds = cc::hip::Since(started);
   });
 }
 
 int cc_hip_graph_solve_neighborhood(cc_hip_graph* graph, const cc_pbils_params* params, int32_t local_search,
                                     int32_t* labels_out, cc_pbils_stats* stats, char* err, size_t err_len) {
-  cc::hip::ClearStats(stats);
+  std::memset(stats, 0, sizeof(*stats));
   return cc::hip::Guard(err, err_len, [&] {
+    const auto started = cc::hip::Clock::now();
     CC_HIP_CHECK(hipSetDevice(graph->g.device));
-    cc::hip::SolveNeighborhood(graph->g, *params, local_search != 0, labels_out, stats);
+    cc::hip::SolveNeighborhood(graph->g, *params, local_search != 0, labels_out, stats, started);
+    stats->seconds = cc::hip::Since(started);
   });
 }
 
 int cc_hip_solve_pbils(const uint64_t* bits, uint32_t n, uint64_t edges, const cc_pbils_params* params,
                        int32_t* labels_out, cc_pbils_stats* stats, int64_t* trace_out, char* err, size_t err_len) {
-  cc::hip::ClearStats(stats);
+  std::memset(stats, 0, sizeof(*stats));
   return cc::hip::Guard(err, err_len, [&] {
     const auto started = cc::hip::Clock::now();  // the solver clock includes allocation and upload
     cc::hip::CheckParams(*params);
     cc::hip::DeviceGraph g;
-    cc::hip::Upload(g, bits, n, edges);
-    cc::hip::SolvePbils(g, *params, labels_out, stats, trace_out);
+    {
+      cc::hip::Stream stream;
+      cc::hip::Upload(g, bits, n, edges, stream.get());
+    }
+    cc::hip::SolvePbils(g, *params, labels_out, stats, trace_out, started);
     stats->seconds = cc::hip::Since(started);
   });
 }
@@ -624,7 +770,7 @@ int cc_hip_solve_pbils(const uint64_t* bits, uint32_t n, uint64_t edges, const c
 int cc_hip_solve_neighborhood(const uint64_t* bits, uint32_t n, uint64_t edges, const cc_pbils_params* params,
                               int32_t local_search, int32_t* labels_out, cc_pbils_stats* stats, char* err,
                               size_t err_len) {
-  cc::hip::ClearStats(stats);
+  std::memset(stats, 0, sizeof(*stats));
   return cc::hip::Guard(err, err_len, [&] {
     const auto started = cc::hip::Clock::now();
     if (params->k != 2) cc::hip::Fail(CC_HIP_EINVAL, "Neighborhood algorithms require k = 2");
@@ -633,9 +779,31 @@ int cc_hip_solve_neighborhood(const uint64_t* bits, uint32_t n, uint64_t edges, 
       cc::hip::Fail(CC_HIP_EINVAL, "Neighborhood requires 1 <= n <= %d", cc::hip::kMaxVerticesWide);
     }
     cc::hip::DeviceGraph g;
-    cc::hip::Upload(g, bits, n, edges);
-    cc::hip::SolveNeighborhood(g, *params, local_search != 0, labels_out, stats);
+    {
+      cc::hip::Stream stream;
+      cc::hip::Upload(g, bits, n, edges, stream.get());
+    }
+    cc::hip::SolveNeighborhood(g, *params, local_search != 0, labels_out, stats, started);
     stats->seconds = cc::hip::Since(started);
+  });
+}
+
+int cc_hip_solve_batch(cc_batch_run* runs, int32_t count, int32_t use_graph, double* seconds, char* err,
+                       size_t err_len) {
+  return cc::hip::Guard(err, err_len, [&] {
+    const auto started = cc::hip::Clock::now();
+    if (count < 1) cc::hip::Fail(CC_HIP_EINVAL, "empty batch");
+    std::vector<cc::hip::BatchRun> batch(count);
+    for (int i = 0; i < count; ++i) {
+      if (!runs[i].graph || !runs[i].labels_out) cc::hip::Fail(CC_HIP_EINVAL, "run %d: graph and labels required", i);
+      std::memset(&runs[i].stats, 0, sizeof(runs[i].stats));
+      batch[i] = {&runs[i].graph->g, runs[i].params, runs[i].labels_out, runs[i].trace_out, &runs[i].stats};
+    }
+    CC_HIP_CHECK(hipSetDevice(runs[0].graph->g.device));
+    cc::hip::SolveBatch(batch, use_graph != 0);
+    const double s = cc::hip::Since(started);
+    for (int i = 0; i < count; ++i) runs[i].stats.seconds = s;
+    if (seconds) *seconds = s;
   });
 }
 

@@ -1,0 +1,205 @@
+// cc_runtime.hpp — host runtime internals shared by cc_runtime.hip (solves, C ABI) and cc_serve.hip (serving).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cc_hip.h"
+#include "cc_internal.hpp"
+
+namespace cc::hip {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& what);
+};
+
+#define CC_HIP_CHECK(call)                                                                                      \
+  do {                                                                                                          \
+    const hipError_t status_ = (call);                                                                          \
+    if (status_ != hipSuccess) {                                                                                \
+      throw ::cc::hip::Error(CC_HIP_EDEVICE, std::string("hip error at ") + __FILE__ + ":" +                    \
+                                                 std::to_string(__LINE__) + ": " + hipGetErrorString(status_)); \
+    }                                                                                                           \
+  } while (0)
+
+[[noreturn]] void Fail(int code, const char* fmt, ...);
+std::string Mebibytes(size_t bytes);
+size_t FreeDeviceBytes();
+uint32_t Words64(uint32_t n);
+
+// Device allocations released together.
+class DeviceMemory {
+ public:
+  DeviceMemory() = default;
+  DeviceMemory(const DeviceMemory&) = delete;
+  DeviceMemory& operator=(const DeviceMemory&) = delete;
+  ~DeviceMemory();
+  void* AllocateBytes(size_t bytes);
+
+ private:
+  std::vector<void*> blocks_;
+};
+
+// A graph on the device: the bitset and the degrees, uploaded once.
+struct DeviceGraph {
+  uint32_t n = 0;
+  uint32_t words = 0;
+  uint64_t edges = 0;
+  int device = 0;
+  uint64_t* bits = nullptr;
+  int32_t* deg = nullptr;
+  DeviceMemory memory;
+};
+
+void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream);
+void CheckParams(const cc_pbils_params& p);
+bool WidePath(uint32_t n, const cc_pbils_params& p);
+DescentPlan PlanDescent(uint32_t n, int k, const cc_pbils_params& p);
+
+using Clock = std::chrono::steady_clock;
+double Since(Clock::time_point t0);
+
+// HIP event pairs around launches on one stream, summed after the next synchronisation.
+class Timer {
+ public:
+  explicit Timer(bool on) : on_(on) {}
+  ~Timer();
+  void Begin(hipStream_t s) {
+    if (on_) Record(s);
+  }
+  void End(hipStream_t s, double* total_ms, int64_t* launches);
+  void Collect();
+
+ private:
+  void Record(hipStream_t s);
+  struct Pending {
+    size_t first;
+    double* total;
+    int64_t* launches;
+  };
+  bool on_;
+  std::vector<hipEvent_t> events_;
+  std::vector<Pending> pending_;
+  size_t used_ = 0;
+};
+
+// A non-blocking stream of default priority.
+class Stream {
+ public:
+  Stream();
+  ~Stream();
+  Stream(const Stream&) = delete;
+  Stream& operator=(const Stream&) = delete;
+  hipStream_t get() const { return s_; }
+
+ private:
+  hipStream_t s_ = nullptr;
+};
+
+// One run of a batch, as the host sees it.
+struct RunSpec {
+  const DeviceGraph* graph;
+  cc_pbils_params params;
+  DescentPlan plan;
+  int population;  // slots (PBILS population, or the Neighborhood batch of starts)
+  bool pbils;      // PBILS: tournament selection and a second population; Neighborhood: descent in place
+};
+
+struct DescentGroup {
+  DescentPlan plan;
+  ItemList list;
+};
+
+// Device state of a set of independent runs: one allocation, the run table, the item tables of every launch and
+// the persistent-queue counters; an iteration of all runs is one fixed launch sequence (captured once as a graph).
+class Batch {
+ public:
+  Batch(std::vector<RunSpec> specs, hipStream_t stream);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  void Start();  // reset, initial labels, rebuild, initial record
+  void EnqueueIteration(Timer* timer, cc_pbils_stats* stats);
+  void Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats);
+  void Rebuild(Timer* timer, cc_pbils_stats* stats);
+  void Descend(Timer* timer, cc_pbils_stats* stats);
+
+  const Ctl* PullCtl();
+  void PullLabels(int run, int32_t* labels_out);
+  void PullTrace(int run, int count, int64_t* out);
+  void SetPopulation(int run, int population);
+
+  const RunDev* Runs() const { return runs_; }
+  const ItemList& AllItems() const { return all_; }
+  const std::vector<RebuildGroup>& RebuildGroups() const { return rebuild_; }
+  const std::vector<DescentGroup>& DescentGroups() const { return descent_; }
+  ItemList Prefix(const ItemList& list, int count) const;
+  DescentPlan Plan(int run) const {
+    for (const DescentGroup& d : descent_) {
+      if (SameLaunch(d.plan, specs_[run].plan)) return d.plan;
+    }
+    return specs_[run].plan;
+  }
+  size_t Bytes() const { return bytes_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  std::vector<RunSpec> specs_;
+  hipStream_t stream_;
+  DeviceMemory memory_;
+  size_t bytes_ = 0;
+  std::vector<RunDev> host_runs_;
+  RunDev* runs_ = nullptr;
+  ItemList all_{};
+  std::vector<RebuildGroup> rebuild_;
+  std::vector<DescentGroup> descent_;
+  int32_t* queues_ = nullptr;
+  int max_n_ = 0;
+  Ctl* host_ctl_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+void RunToCompletion(Batch& batch, const std::vector<RunSpec>& specs, bool verbose, bool use_graph, double time_limit,
+                     Timer* timer, cc_pbils_stats* stats, Clock::time_point started);
+
+void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_out, cc_pbils_stats* stats,
+                int64_t* trace_out, Clock::time_point started);
+void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool local_search, int32_t* labels_out,
+                       cc_pbils_stats* stats, Clock::time_point started);
+
+struct BatchRun {
+  const DeviceGraph* graph;
+  cc_pbils_params params;
+  int32_t* labels;
+  int64_t* trace;
+  cc_pbils_stats* stats;
+};
+void SolveBatch(std::vector<BatchRun>& runs, bool use_graph);
+
+int Report(const std::exception& e, char* err, size_t err_len, int code);
+
+template <typename Body>
+int Guard(char* err, size_t err_len, Body&& body) {
+  try {
+    body();
+    return CC_HIP_OK;
+  } catch (const Error& e) {
+    return Report(e, err, err_len, e.code);
+  } catch (const std::exception& e) {
+    return Report(e, err, err_len, CC_HIP_EDEVICE);
+  }
+}
+
+}  // namespace cc::hip
+
+struct cc_hip_graph {
+  cc::hip::DeviceGraph g;
+};

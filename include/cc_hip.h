@@ -114,6 +114,22 @@ int cc_hip_graph_solve_pbils(cc_hip_graph* graph, const cc_pbils_params* params,
 int cc_hip_graph_solve_neighborhood(cc_hip_graph* graph, const cc_pbils_params* params, int32_t local_search,
                                     int32_t* labels_out, cc_pbils_stats* stats, char* err, size_t err_len);
 
+/* ---- batched solves: many independent PBILS runs (different graphs, n, k and parameters) advanced together by
+ *      one launch per phase, descents claimed from persistent work queues, iterations replayed from a HIP graph.
+ *      Each run's result is bit-identical to its own cc_hip_graph_solve_pbils. No reference counterpart: the
+ *      reference solves one instance per call. --------------------------------------------------------------- */
+
+typedef struct cc_batch_run {
+  const cc_hip_graph* graph;
+  cc_pbils_params params;
+  int32_t* labels_out;  /* n entries */
+  int64_t* trace_out;   /* nullable: params.iterations entries */
+  cc_pbils_stats stats; /* filled; stats.seconds is the wall time of the whole batch */
+} cc_batch_run;
+
+int cc_hip_solve_batch(cc_batch_run* runs, int32_t count, int32_t use_graph, double* seconds, char* err,
+                       size_t err_len);
+
 /* ---- host-side graph generators and recounts of the product (the reference's Graph::ErdosRenyi,
  *      src/cc_graph.cpp:168-177, and the recounts of src/cc_pbils_cpu.cpp:12-53) --------------------------- */
 

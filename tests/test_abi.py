@@ -61,6 +61,20 @@ def test_product_recounts_match_oracle(orc):
             assert cga.objective_bitwise(g, labels) == want
 
 
+def test_workload_streams_match_the_rng_contract(orc):
+    from cuda_graph_clustering_amd import workloads
+
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        seed, stream, step = (int(x) for x in rng.integers(0, 2**63, 3, dtype=np.uint64))
+        assert workloads.stream_seed(seed, stream, step) == orc.stream_seed(seed, stream, step)
+    a = workloads.mixed_instances(64)
+    assert a == workloads.mixed_instances(64)
+    ns = [x["gen"][1] for x in a]
+    assert min(ns) >= 200 and max(ns) <= 8000
+    assert {x["k"] for x in a} <= set(range(2, 9)) and len({x["gen"][0] for x in a}) == 2
+
+
 def test_cli_binary_is_built():
     import cuda_graph_clustering_amd as cga
 
