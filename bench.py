@@ -95,6 +95,55 @@ def cpu_sample(cfg, graph, dg, threads):
     return out
 
 
+def percentile(values, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q))
+
+
+def bench_batch(cga, count, population=64, iterations=50, master=7):
+    """The mixed stream: `count` independent instances (n log-uniform in [200, 8000], ER or SBM, k in 2..8), pop 64,
+    50 iterations, gww 0.1, seeds from master seed 7; solved plain (one instance per solve call, one after the
+    other) and batched (one cc_hip_solve_batch), with every batched result compared to its plain result."""
+    from cuda_graph_clustering_amd import workloads
+
+    insts = workloads.mixed_instances(count, master=master)
+    graphs = [cga.DeviceGraph(workloads.build_graph(cga, inst)) for inst in insts]
+    params = [dict(k=inst["k"], population=population, iterations=iterations, early_stop=iterations, gww=0.1,
+                   seed=inst["seed"]) for inst in insts]
+    cga.solve_batch(graphs[:4], params[:4])  # warm-up: kernels loaded, graph path exercised
+    plain, latencies = [], []
+    t0 = time.perf_counter()
+    for g, kw in zip(graphs, params):
+        t1 = time.perf_counter()
+        plain.append(g.solve_pbils(**kw))
+        latencies.append(time.perf_counter() - t1)
+    plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched, _ = cga.solve_batch(graphs, params)
+    batch_s = time.perf_counter() - t0
+    exact = all(a["objective"] == b["objective"] and a["accepted_moves"] == b["accepted_moves"] and
+                a["iterations_done"] == b["iterations_done"] and bool((a["labels"] == b["labels"]).all())
+                for a, b in zip(plain, batched))
+    for g in graphs:
+        g.close()
+    return {
+        "workload": f"{count} instances, n log-uniform [200, 8000], ER p~U[0.05,0.5) or SBM 2-8 blocks "
+                    f"(0.4/0.05) 50/50, k~U{{2..8}}, pop {population}, {iterations} iterations, gww 0.1, "
+                    f"master seed {master}",
+        "batched_graphs_per_s": round(count / batch_s, 3),
+        "plain_graphs_per_s": round(count / plain_s, 3),
+        "speedup": round(plain_s / batch_s, 2),
+        "batched_seconds": round(batch_s, 3),
+        "plain_seconds": round(plain_s, 3),
+        "plain_latency_p50_s": round(percentile(latencies, 50), 4),
+        "plain_latency_p99_s": round(percentile(latencies, 99), 4),
+        "batched_latency_s": round(batch_s, 3),
+        "bit_exact": bool(exact),
+        "moves": int(sum(r["accepted_moves"] for r in batched)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +151,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -183,6 +233,7 @@ def main():
         }
     else:
         out["cpu_baseline"] = None
+    out["batched"] = bench_batch(cga, args.batch_count) if args.batch_count > 0 else None
     print(json.dumps(out))
 
 

@@ -766,6 +766,47 @@ __global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, Ite
   Persistent(runs, list, [](const RunDev& r, int s) { DescentK3WideBody<T>(r, s); });
 }
 
+// The mixed descent: one persistent 1024-thread launch for a batch whose runs use different register kernels. The
+// queue holds the items of all of them, largest runs first, and each item runs the variant its run names
+// (RunDev::variant), so no CU idles while another kernel instance's launch drains its tail.
+struct MixedDesc {
+  int kind;  // 0: k = 2 registers, 1: k in [3, 8] registers
+  int k;
+  int vpt;
+};
+constexpr MixedDesc kMixedTable[] = {
+    {0, 2, 1},  {0, 2, 2},  {0, 2, 4},  {0, 2, 8},  {0, 2, 16}, {0, 2, 32},             //
+    {1, 3, 1},  {1, 3, 2},  {1, 3, 4},  {1, 3, 8},  {1, 3, 12},                          //
+    {1, 4, 1},  {1, 4, 2},  {1, 4, 4},  {1, 4, 8},  {1, 4, 12},                          //
+    {1, 5, 1},  {1, 5, 2},  {1, 5, 4},  {1, 5, 8},                                       //
+    {1, 6, 1},  {1, 6, 2},  {1, 6, 4},  {1, 6, 8},                                       //
+    {1, 7, 1},  {1, 7, 2},  {1, 7, 4},  {1, 7, 8},                                       //
+    {1, 8, 1},  {1, 8, 2},  {1, 8, 4},  {1, 8, 8},                                       //
+};
+constexpr int kMixedCount = sizeof(kMixedTable) / sizeof(kMixedTable[0]);
+constexpr int kMixedThreads = 1024;
+
+template <int I>
+__device__ __forceinline__ void MixedOne(const RunDev& r, int s) {
+  constexpr MixedDesc d = kMixedTable[I];
+  if constexpr (d.kind == 0) {
+    DescentK2Body<kMixedThreads, d.vpt, false>(r, s);
+  } else {
+    DescentRegsBody<kMixedThreads, d.k, d.vpt>(r, s);
+  }
+}
+
+template <int... I>
+__device__ __forceinline__ void MixedSwitch(const RunDev& r, int s, std::integer_sequence<int, I...>) {
+  const int v = r.variant;
+  ((v == I ? (MixedOne<I>(r, s), 0) : 0), ...);
+}
+
+__global__ void __launch_bounds__(kMixedThreads) KernelDescentMixed(const RunDev* runs, ItemList list) {
+  Persistent(runs, list,
+             [](const RunDev& r, int s) { MixedSwitch(r, s, std::make_integer_sequence<int, kMixedCount>{}); });
+}
+
 template <typename Kernel>
 void AllowLds(Kernel kernel, size_t bytes) {
   if (bytes > 0) {
@@ -919,6 +960,8 @@ bool Dispatch(const Go& go) {
       return p.wide && p.k == 2 && GoWide<WideK2>(go);
     case Descent::kK3Wide:
       return p.wide && p.k == 3 && GoWide<WideK3>(go);
+    case Descent::kMixed:
+      return !p.wide && p.threads == kMixedThreads && go.Run(KernelDescentMixed, kMixedThreads, 0);
   }
   return false;
 }
@@ -939,6 +982,23 @@ void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan&
 
 size_t GenericLdsBytes(int n, int k, int wide) {
   return Align16((wide ? 4 : 2) * static_cast<size_t>(k) * n) + Align16(static_cast<size_t>(n));
+}
+
+int MixedVariant(int n, int k, DescentPlan* plan) {
+  for (int i = 0; i < kMixedCount; ++i) {
+    const MixedDesc& d = kMixedTable[i];
+    if (d.k == k && d.vpt * kMixedThreads >= n) {
+      plan->descent = d.kind == 0 ? Descent::kK2Registers : Descent::kRegisters;
+      plan->k = k;
+      plan->threads = kMixedThreads;
+      plan->vpt = d.vpt;
+      plan->rows_in_lds = false;
+      plan->lds_bytes = 0;
+      plan->wide = 0;
+      return i;
+    }
+  }
+  return -1;
 }
 
 int RegistersVpt(int n, int k, int* threads) {

@@ -66,6 +66,7 @@ struct RunDev {
   int32_t do_perturb;  // k >= 2 and perturbation > 0 (src/cc_pbils_cpu.cpp:241)
   int32_t select;      // 1: the descent of slot s starts from Tournament(pop); 0: from pop[s] itself
   int32_t wide;        // 32-bit G
+  int32_t variant;     // index of the run's kernel in the mixed descent (register kernels, 1024 threads), or -1
 };
 
 // A slot of a run: the unit of work of the slot-level kernels and of the descent.
@@ -125,6 +126,7 @@ enum class Descent : int {
   kK2Wide = 3,
   kK3Wide = 4,
   kRegisters = 5,  // k in [3, 8], narrow, state in registers
+  kMixed = 6,      // a batch launch over runs of different register kernels (1024 threads), by RunDev::variant
 };
 
 struct DescentPlan {
@@ -150,5 +152,8 @@ bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 int RegistersVpt(int n, int k, int* threads);
+// The mixed-descent variant that runs a register plan of (k, n) with 1024 threads, or -1 if none; `plan` receives
+// the equivalent 1024-thread plan.
+int MixedVariant(int n, int k, DescentPlan* plan);
 
 }  // namespace cc::hip

@@ -362,6 +362,7 @@ Batch::Batch(std::vector<RunSpec> specs, hipStream_t stream) : specs_(std::move(
     r.do_perturb = (p.k >= 2 && p.perturbation > 0.0) ? 1 : 0;
     r.select = s.pbils ? 1 : 0;
     r.wide = s.plan.wide;
+    r.variant = s.variant;
   }
   CC_HIP_CHECK(hipMemcpyAsync(runs_, host_runs_.data(), sizeof(RunDev) * nruns, hipMemcpyHostToDevice, stream_));
   all_.count = static_cast<int32_t>(all.size());
@@ -507,6 +508,7 @@ const char* DescentName(Descent d) {
     case Descent::kK2Wide: return "k = 2, packed words";
     case Descent::kK3Wide: return "k = 3, packed words";
     case Descent::kRegisters: return "k = 3..8, registers";
+    case Descent::kMixed: return "mixed register kernels";
   }
   return "?";
 }
@@ -669,7 +671,20 @@ void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
   std::vector<RunSpec> specs;
   for (BatchRun& b : runs) {
     CheckParams(b.params);
-    specs.push_back({b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params), b.params.population, true});
+    RunSpec spec{b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params), b.params.population, true};
+    // Register-kernel runs of a batch share the mixed launch (same trajectory, 1024 threads per workgroup).
+    const bool registers = spec.plan.descent == Descent::kK2Registers || spec.plan.descent == Descent::kRegisters;
+    if (runs.size() > 1 && registers && b.params.ls_kernel == CC_LS_AUTO) {
+      DescentPlan variant_plan;
+      const int variant = MixedVariant(static_cast<int>(b.graph->n), b.params.k, &variant_plan);
+      if (variant >= 0) {
+        spec.variant = variant;
+        spec.plan = DescentPlan{};
+        spec.plan.descent = Descent::kMixed;
+        spec.plan.threads = 1024;
+      }
+    }
+    specs.push_back(spec);
   }
   Stream stream;
   Batch batch(specs, stream.get());

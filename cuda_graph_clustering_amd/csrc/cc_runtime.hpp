@@ -109,6 +109,7 @@ struct RunSpec {
   DescentPlan plan;
   int population;  // slots (PBILS population, or the Neighborhood batch of starts)
   bool pbils;      // PBILS: tournament selection and a second population; Neighborhood: descent in place
+  int variant = -1;  // mixed-descent variant when plan.descent is kMixed
 };
 
 struct DescentGroup {
