@@ -144,6 +144,78 @@ def bench_batch(cga, count, population=64, iterations=50, master=7):
     }
 
 
+def bench_serve(cga, rate=50.0, pool=96, big_iterations=20, check=16, master=7):
+    """Serving with continuous batching: the wide-path instance (Chung-Lu n=100000, gamma 2.5, mean degree 64,
+    graph-seed 4; k=3, pop 128, 20 iterations) on the large lane, beside a Poisson stream (rate req/s) of requests
+    drawn from the mixed-batch distribution (pop 64, 50 iterations) submitted while the large run is on the device.
+    The stream cycles over `pool` pre-generated graphs, each request with its own search seed."""
+    import threading
+
+    import numpy as np
+    from cuda_graph_clustering_amd import workloads
+
+    insts = workloads.mixed_instances(pool, master=master)
+    graphs = [workloads.build_graph(cga, inst) for inst in insts]
+    big = cga.chung_lu(100000, 2.5, 64.0, 4)
+    big_kw = dict(k=3, population=128, iterations=big_iterations, early_stop=big_iterations, gww=0.1, seed=42)
+    arrivals = np.random.default_rng(master).exponential(1.0 / rate, size=100000)
+    submitted = []
+    stop = threading.Event()
+    with cga.Server(lanes=2, large_bytes=256 << 20) as server:
+        t0 = time.perf_counter()
+        big_id = server.submit(big, **big_kw)
+
+        def feed():
+            when = 0.0
+            for i, gap in enumerate(arrivals):
+                when += gap
+                delay = t0 + when - time.perf_counter()
+                if delay > 0 and stop.wait(delay):
+                    return
+                if stop.is_set():
+                    return
+                inst = insts[i % pool]
+                kw = dict(k=inst["k"], population=64, iterations=50, early_stop=50, gww=0.1,
+                          seed=workloads.stream_seed(master, workloads.BATCH_SEARCH_STREAM, i))
+                submitted.append((server.submit(graphs[i % pool], **kw), i % pool, kw))
+
+        feeder = threading.Thread(target=feed)
+        feeder.start()
+        big_res = server.wait(big_id)
+        stop.set()
+        feeder.join()
+        stream_s = time.perf_counter() - t0
+        results = [(server.wait(rid), gi, kw) for rid, gi, kw in submitted]
+        total_s = time.perf_counter() - t0
+        counters = server.counters()
+    exact = True
+    for res, gi, kw in results[:check]:
+        with cga.DeviceGraph(graphs[gi]) as dg:
+            solo = dg.solve_pbils(**kw)
+        exact &= (res["objective"] == solo["objective"] and res["accepted_moves"] == solo["accepted_moves"] and
+                  bool((res["labels"] == solo["labels"]).all()))
+    lat = [r["latency"] for r, _, _ in results]
+    queued = [r["queued"] for r, _, _ in results]
+    return {
+        "workload": f"wide-path run Chung-Lu n=100000 (gamma 2.5, mean degree 64, graph-seed 4) k=3 pop 128 "
+                    f"{big_iterations} iterations on lane 0, beside a Poisson stream at {rate:g} req/s of mixed-batch "
+                    f"requests (pop 64, 50 iterations) over {pool} graphs, submitted while the wide run is active",
+        "requests": len(results),
+        "graphs_per_s": round(len(results) / total_s, 3) if results else 0.0,
+        "latency_p50_s": round(percentile(lat, 50), 3) if lat else None,
+        "latency_p99_s": round(percentile(lat, 99), 3) if lat else None,
+        "queued_p50_s": round(percentile(queued, 50), 3) if queued else None,
+        "wide_run_latency_s": round(big_res["latency"], 3),
+        "wide_run_objective": big_res["objective"],
+        "stream_seconds": round(stream_s, 3),
+        "drain_seconds": round(total_s - stream_s, 3),
+        "peak_active": counters["peak_active"],
+        "peak_bytes": counters["peak_bytes"],
+        "bit_exact_sample": bool(exact),
+        "checked": min(check, len(results)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +224,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson request rate; 0 skips serving")
+    ap.add_argument("--serve-iterations", type=int, default=20, help="iterations of the wide-path served run")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -234,6 +308,8 @@ def main():
     else:
         out["cpu_baseline"] = None
     out["batched"] = bench_batch(cga, args.batch_count) if args.batch_count > 0 else None
+    out["serving"] = (bench_serve(cga, rate=args.serve_rate, big_iterations=args.serve_iterations)
+                      if args.serve_rate > 0 else None)
     print(json.dumps(out))
 
 

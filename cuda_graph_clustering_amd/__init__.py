@@ -84,6 +84,15 @@ class BatchRun(ctypes.Structure):
     ]
 
 
+class ServeOptions(ctypes.Structure):
+    _fields_ = [
+        ("budget_bytes", ctypes.c_int64),
+        ("lanes", ctypes.c_int32),
+        ("max_active", ctypes.c_int32),
+        ("large_bytes", ctypes.c_int64),
+    ]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -111,6 +120,13 @@ ABI_SYMBOLS = [
     "cc_hip_graph_solve_pbils",
     "cc_hip_graph_solve_neighborhood",
     "cc_hip_solve_batch",
+    "cc_serve_default_options",
+    "cc_hip_server_create",
+    "cc_hip_server_destroy",
+    "cc_hip_server_submit",
+    "cc_hip_server_wait",
+    "cc_hip_footprint",
+    "cc_hip_server_counters",
     "cc_words64",
     "cc_graph_erdos_renyi",
     "cc_graph_sbm",
@@ -140,6 +156,19 @@ def _load():
                                                     ctypes.POINTER(Stats), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_hip_solve_batch.argtypes = [ctypes.POINTER(BatchRun), ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_serve_default_options.argtypes = [ctypes.POINTER(ServeOptions)]
+    lib.cc_hip_server_create.argtypes = [ctypes.POINTER(ServeOptions), ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_server_destroy.argtypes = [ctypes.c_void_p]
+    lib.cc_hip_server_submit.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.c_uint64,
+                                         ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p,
+                                         ctypes.c_size_t]
+    lib.cc_hip_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, _i32p, _i64p,
+                                       ctypes.POINTER(Stats), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_footprint.argtypes = [ctypes.c_uint32, ctypes.POINTER(Params)]
+    lib.cc_hip_footprint.restype = ctypes.c_int64
+    lib.cc_hip_server_counters.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 5
     lib.cc_words64.argtypes = [ctypes.c_uint32]
     lib.cc_words64.restype = ctypes.c_uint32
     lib.cc_graph_erdos_renyi.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, _u64p, ctypes.c_int32]
@@ -331,6 +360,72 @@ def solve_batch(device_graphs, params, use_graph=True):
     _check(rc, err)
     results = [_result(runs[i].stats, keep[i][0], keep[i][2], keep[i][1]) for i in range(count)]
     return results, seconds.value
+
+
+def footprint(n, **kw):
+    """Device bytes a served solve of an n-vertex graph holds: bitset and degrees plus the run's state."""
+    p = make_params(**kw)
+    return lib.cc_hip_footprint(n, ctypes.byref(p))
+
+
+class Server:
+    """Serving with continuous batching (cc_hip_server_*): submit() returns at once; wait() returns the result with
+    its latency (submit to result) and queueing time (submit to admission)."""
+
+    def __init__(self, budget_bytes=0, lanes=2, max_active=4096, large_bytes=1 << 30):
+        o = ServeOptions()
+        lib.cc_serve_default_options(ctypes.byref(o))
+        o.budget_bytes, o.lanes, o.max_active, o.large_bytes = budget_bytes, lanes, max_active, large_bytes
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        _check(lib.cc_hip_server_create(ctypes.byref(o), ctypes.byref(handle), err, 512), err)
+        self.handle = handle
+        self.shapes = {}
+
+    def submit(self, graph, **kw):
+        p = make_params(**kw)
+        rid = ctypes.c_uint64()
+        err = ctypes.create_string_buffer(512)
+        _check(lib.cc_hip_server_submit(self.handle, graph.ptr(), graph.n, graph.edges, ctypes.byref(p),
+                                        ctypes.byref(rid), err, 512), err)
+        self.shapes[rid.value] = (graph.n, p.iterations)
+        return rid.value
+
+    def wait(self, rid, timeout=0.0):
+        n, iterations = self.shapes[rid]
+        labels = np.zeros(max(n, 1), dtype=np.int32)
+        trace = np.zeros(max(iterations, 1), dtype=np.int64)
+        stats = Stats()
+        latency, queued = ctypes.c_double(), ctypes.c_double()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.cc_hip_server_wait(self.handle, rid, timeout, labels.ctypes.data_as(_i32p),
+                                    trace.ctypes.data_as(_i64p), ctypes.byref(stats), ctypes.byref(latency),
+                                    ctypes.byref(queued), err, 512)
+        _check(rc, err)
+        del self.shapes[rid]
+        out = _result(stats, labels, n, trace)
+        out["latency"], out["queued"] = latency.value, queued.value
+        return out
+
+    def counters(self):
+        vals = [ctypes.c_int64() for _ in range(5)]
+        lib.cc_hip_server_counters(self.handle, *[ctypes.byref(v) for v in vals])
+        return dict(zip(("admitted", "completed", "budget_left", "peak_active", "peak_bytes"),
+                        (v.value for v in vals)))
+
+    def close(self):
+        if self.handle:
+            lib.cc_hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def device_info():

@@ -8,6 +8,7 @@
 // stop fires.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "cc_internal.hpp"
@@ -326,6 +327,15 @@ __global__ void __launch_bounds__(kBlock) KernelNeighborhoodLabels(const RunDev*
 
 int Blocks(int count, int per) { return (count + per - 1) / per; }
 
+// Kernels that take their item from blockIdx.y are launched in chunks of at most kMaxGridY items, each with the
+// item table advanced to its first item.
+constexpr int kMaxGridY = 65535;
+
+template <typename Launch>
+void ForChunksY(const Item* items, int count, Launch&& launch) {
+  for (int first = 0; first < count; first += kMaxGridY) launch(items + first, std::min(kMaxGridY, count - first));
+}
+
 }  // namespace
 
 void LaunchResetCtl(const RunDev* runs, int nruns, hipStream_t stream) {
@@ -341,8 +351,9 @@ void LaunchDegrees(const RunDev* run, int n, hipStream_t stream) {
 }
 
 void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream) {
-  if (items.count == 0) return;
-  KernelInitLabels<<<dim3(Blocks(max_n, kBlock), items.count), kBlock, 0, stream>>>(runs, items.items);
+  ForChunksY(items.items, items.count, [&](const Item* chunk, int count) {
+    KernelInitLabels<<<dim3(Blocks(max_n, kBlock), count), kBlock, 0, stream>>>(runs, chunk);
+  });
 }
 
 void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream) {
@@ -352,15 +363,17 @@ void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream) 
 
 template <typename GT>
 static void LaunchRebuildGT(const RunDev* runs, const RebuildGroup& g, int max_n, hipStream_t stream) {
-  const dim3 grid(Blocks(max_n, kRebuildRows), g.pairs.count, 1);
-  if (g.kc == 2) {
-    KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
-  } else if (g.kc == 4) {
-    KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
-  } else {
-    const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
-    KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, g.pairs.items);
-  }
+  ForChunksY(g.pairs.items, g.pairs.count, [&](const Item* chunk, int count) {
+    const dim3 grid(Blocks(max_n, kRebuildRows), count, 1);
+    if (g.kc == 2) {
+      KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, chunk);
+    } else if (g.kc == 4) {
+      KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, chunk);
+    } else {
+      const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
+      KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, chunk);
+    }
+  });
 }
 
 void LaunchRebuildG(const RunDev* runs, const RebuildGroup& group, int max_n, hipStream_t stream) {
@@ -382,8 +395,9 @@ void LaunchRecord(const RunDev* runs, int nruns, int mode, int read_next, hipStr
 }
 
 void LaunchPerturb(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream) {
-  if (items.count == 0) return;
-  KernelPerturb<<<dim3(Blocks(max_n, kBlock), items.count), kBlock, 0, stream>>>(runs, items.items);
+  ForChunksY(items.items, items.count, [&](const Item* chunk, int count) {
+    KernelPerturb<<<dim3(Blocks(max_n, kBlock), count), kBlock, 0, stream>>>(runs, chunk);
+  });
 }
 
 void LaunchNeighborhoodLabels(const RunDev* run, int n, int first, int count, hipStream_t stream) {

@@ -208,7 +208,7 @@ void Timer::Collect() {
 Stream::Stream() { CC_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
 Stream::~Stream() { (void)hipStreamDestroy(s_); }
 
-// ---- the batch engine ------------------------------------------------------------------------------------------
+// ---- run state and launch sets --------------------------------------------------------------------------------
 
 namespace {
 
@@ -219,8 +219,7 @@ class Arena {
   template <typename T>
   void Take(T*& ptr, size_t count) {
     const size_t bytes = (count * sizeof(T) + 255) & ~static_cast<size_t>(255);
-    ptr = base_ ? reinterpret_cast<T*>(base_ + used_) : nullptr;
-    if (count == 0) ptr = nullptr;
+    ptr = (base_ && count) ? reinterpret_cast<T*>(base_ + used_) : nullptr;
     used_ += bytes;
   }
   size_t used() const { return used_; }
@@ -242,20 +241,102 @@ double DescentCost(const RunSpec& s) {
   return n * n * static_cast<double>(s.params.k);
 }
 
+// The buffers of one run (reference Run::ForEachBuffer, cuda/cc_pbils_gpu.cu:165-184).
+void LayoutRun(const RunSpec& s, RunDev& r, Arena& a) {
+  const size_t n = s.graph->n, k = static_cast<size_t>(s.params.k), P = static_cast<size_t>(s.population);
+  const size_t iterations = static_cast<size_t>(std::max(s.params.iterations, 1));
+  auto pop = [&](PopDev& d) {
+    a.Take(d.labels, P * n);
+    if (s.plan.wide) {
+      int32_t* g = nullptr;
+      a.Take(g, P * k * n);
+      d.g = g;
+    } else {
+      int16_t* g = nullptr;
+      a.Take(g, P * k * n);
+      d.g = g;
+    }
+    a.Take(d.sizes, P * k);
+    a.Take(d.f, P);
+  };
+  pop(r.pop);
+  if (s.pbils) {
+    pop(r.next);
+  } else {
+    r.next = r.pop;
+  }
+  a.Take(r.moves, P);
+  a.Take(r.ctl, 1);
+  a.Take(r.rec_labels, n);
+  a.Take(r.trace, iterations);
+  a.Take(r.src, P);
+  a.Take(r.order, P);
+  a.Take(r.masks, P * k * s.graph->words);
+  a.Take(r.intra, P);
+  a.Take(r.work, s.plan.descent == Descent::kK3Wide ? P * n : 0);
+}
+
 }  // namespace
 
-Batch::Batch(std::vector<RunSpec> specs, hipStream_t stream) : specs_(std::move(specs)), stream_(stream) {
-  const int nruns = static_cast<int>(specs_.size());
+size_t RunState::Footprint(const RunSpec& spec) {
+  Arena a;
+  RunDev r{};
+  LayoutRun(spec, r, a);
+  return a.used();
+}
+
+RunState::RunState(const RunSpec& spec, hipStream_t stream) : spec_(spec) {
+  bytes_ = Footprint(spec_);
+  const size_t free_bytes = FreeDeviceBytes();
+  if (bytes_ > free_bytes) {
+    Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes_).c_str(),
+         Mebibytes(free_bytes).c_str());
+  }
+  uint8_t* base = static_cast<uint8_t*>(memory_.AllocateBytes(bytes_));
+  // Start from zeros: e.g. a Neighborhood batch without local search sums a moves[] that no descent writes.
+  CC_HIP_CHECK(hipMemsetAsync(base, 0, bytes_, stream));
+  Arena arena;
+  arena.Start(base);
+  RunDev& r = dev_;
+  LayoutRun(spec_, r, arena);
+  const cc_pbils_params& p = spec_.params;
+  r.bits = spec_.graph->bits;
+  r.deg = spec_.graph->deg;
+  r.n = static_cast<int32_t>(spec_.graph->n);
+  r.words = static_cast<int32_t>(spec_.graph->words);
+  r.k = p.k;
+  r.population = spec_.population;
+  r.edges = static_cast<int64_t>(spec_.graph->edges);
+  r.init_stream = StreamSeed(p.seed, kInitStream, 0);
+  r.select_stream = StreamSeed(p.seed, kSelectStream, 0);
+  r.perturb_stream = StreamSeed(p.seed, kPerturbStream, 0);
+  r.tournament = p.tournament;
+  r.copies = spec_.pbils ? static_cast<int32_t>(p.gww * p.population) : 0;  // WinnerCopies, src/cc_pbils_cpu.cpp:168
+  r.early_stop = p.early_stop;
+  r.iterations = p.iterations;
+  r.perturbation = static_cast<float>(p.perturbation);
+  r.do_perturb = (p.k >= 2 && p.perturbation > 0.0) ? 1 : 0;
+  r.select = spec_.pbils ? 1 : 0;
+  r.wide = spec_.plan.wide;
+  r.variant = spec_.variant;
+}
+
+LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream) : states_(states), stream_(stream) {
+  const int nruns = static_cast<int>(states_.size());
   if (nruns == 0) Fail(CC_HIP_EINVAL, "empty batch");
   host_runs_.resize(nruns);
-  for (const RunSpec& s : specs_) max_n_ = std::max(max_n_, static_cast<int>(s.graph->n));
+  for (int i = 0; i < nruns; ++i) {
+    host_runs_[i] = states_[i]->dev();
+    max_n_ = std::max(max_n_, static_cast<int>(states_[i]->spec().graph->n));
+  }
+  auto spec = [&](int i) -> const RunSpec& { return states_[i]->spec(); };
 
   // item tables (host)
   std::vector<Item> all;
-  std::map<std::pair<int, int>, std::vector<Item>> pairs;  // (wide, kc) -> pairs
+  std::map<std::pair<int, int>, std::vector<Item>> pairs;  // (wide, kc) -> slot pairs
   std::vector<std::pair<DescentPlan, std::vector<Item>>> groups;
   for (int i = 0; i < nruns; ++i) {
-    const RunSpec& s = specs_[i];
+    const RunSpec& s = spec(i);
     for (int slot = 0; slot < s.population; ++slot) all.push_back({i, slot});
     auto& pl = pairs[{s.plan.wide, KcOf(s.params.k)}];
     for (int slot = 0; slot < s.population; slot += 2) pl.push_back({i, slot});
@@ -266,142 +347,63 @@ Batch::Batch(std::vector<RunSpec> specs, hipStream_t stream) : specs_(std::move(
     for (int slot = 0; slot < s.population; ++slot) groups[g].second.push_back({i, slot});
   }
   for (auto& g : groups) {
-    std::stable_sort(g.second.begin(), g.second.end(), [&](const Item& a, const Item& b) {
-      return DescentCost(specs_[a.run]) > DescentCost(specs_[b.run]);
-    });
+    std::stable_sort(g.second.begin(), g.second.end(),
+                     [&](const Item& a, const Item& b) { return DescentCost(spec(a.run)) > DescentCost(spec(b.run)); });
   }
 
-  // layout: per run buffers, then the item tables and queues
+  // one allocation for the tables
+  std::vector<Item*> pair_tables(pairs.size()), group_tables(groups.size());
+  Item* all_table = nullptr;
   auto layout = [&](Arena& a) {
-    for (int i = 0; i < nruns; ++i) {
-      const RunSpec& s = specs_[i];
-      RunDev& r = host_runs_[i];
-      const size_t n = s.graph->n, k = static_cast<size_t>(s.params.k), P = static_cast<size_t>(s.population);
-      const size_t iterations = static_cast<size_t>(std::max(s.params.iterations, 1));
-      auto pop = [&](PopDev& d) {
-        a.Take(d.labels, P * n);
-        if (s.plan.wide) {
-          int32_t* g = nullptr;
-          a.Take(g, P * k * n);
-          d.g = g;
-        } else {
-          int16_t* g = nullptr;
-          a.Take(g, P * k * n);
-          d.g = g;
-        }
-        a.Take(d.sizes, P * k);
-        a.Take(d.f, P);
-      };
-      pop(r.pop);
-      if (s.pbils) {
-        pop(r.next);
-      } else {
-        r.next = r.pop;
-      }
-      a.Take(r.moves, P);
-      a.Take(r.ctl, 1);
-      a.Take(r.rec_labels, n);
-      a.Take(r.trace, iterations);
-      a.Take(r.src, P);
-      a.Take(r.order, P);
-      a.Take(r.masks, P * k * s.graph->words);
-      a.Take(r.intra, P);
-      a.Take(r.work, s.plan.descent == Descent::kK3Wide ? P * n : 0);
-    }
     a.Take(runs_, nruns);
-    a.Take(all_.items, all.size());
-    for (auto& [key, list] : pairs) {
-      RebuildGroup g{};
-      a.Take(const_cast<Item*&>(g.pairs.items), list.size());
-      rebuild_.push_back(g);
-    }
-    for (auto& g : groups) {
-      ItemList l{};
-      a.Take(const_cast<Item*&>(l.items), g.second.size());
-      descent_.push_back({g.first, l});
-    }
+    a.Take(all_table, all.size());
+    size_t i = 0;
+    for (auto& kv : pairs) a.Take(pair_tables[i++], kv.second.size());
+    for (size_t g = 0; g < groups.size(); ++g) a.Take(group_tables[g], groups[g].second.size());
     a.Take(queues_, std::max<size_t>(groups.size(), 1));
   };
   Arena sizing;
   layout(sizing);
-  rebuild_.clear();
-  descent_.clear();
-  bytes_ = sizing.used();
-  const size_t free_bytes = FreeDeviceBytes();
-  if (bytes_ > free_bytes) {
-    Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes_).c_str(),
-         Mebibytes(free_bytes).c_str());
-  }
-  uint8_t* base = static_cast<uint8_t*>(memory_.AllocateBytes(bytes_));
-  // Start from zeros: e.g. a Neighborhood batch without local search sums a moves[] that no descent writes.
-  CC_HIP_CHECK(hipMemsetAsync(base, 0, bytes_, stream_));
   Arena arena;
-  arena.Start(base);
+  arena.Start(static_cast<uint8_t*>(memory_.AllocateBytes(sizing.used())));
   layout(arena);
 
-  // run descriptors
-  for (int i = 0; i < nruns; ++i) {
-    const RunSpec& s = specs_[i];
-    RunDev& r = host_runs_[i];
-    const cc_pbils_params& p = s.params;
-    r.bits = s.graph->bits;
-    r.deg = s.graph->deg;
-    r.n = static_cast<int32_t>(s.graph->n);
-    r.words = static_cast<int32_t>(s.graph->words);
-    r.k = p.k;
-    r.population = s.population;
-    r.edges = static_cast<int64_t>(s.graph->edges);
-    r.init_stream = StreamSeed(p.seed, kInitStream, 0);
-    r.select_stream = StreamSeed(p.seed, kSelectStream, 0);
-    r.perturb_stream = StreamSeed(p.seed, kPerturbStream, 0);
-    r.tournament = p.tournament;
-    r.copies = s.pbils ? static_cast<int32_t>(p.gww * p.population) : 0;  // WinnerCopies, src/cc_pbils_cpu.cpp:168
-    r.early_stop = p.early_stop;
-    r.iterations = p.iterations;
-    r.perturbation = static_cast<float>(p.perturbation);
-    r.do_perturb = (p.k >= 2 && p.perturbation > 0.0) ? 1 : 0;
-    r.select = s.pbils ? 1 : 0;
-    r.wide = s.plan.wide;
-    r.variant = s.variant;
-  }
   CC_HIP_CHECK(hipMemcpyAsync(runs_, host_runs_.data(), sizeof(RunDev) * nruns, hipMemcpyHostToDevice, stream_));
-  all_.count = static_cast<int32_t>(all.size());
-  CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(all_.items), all.data(), sizeof(Item) * all.size(),
-                              hipMemcpyHostToDevice, stream_));
-  size_t gi = 0;
+  all_ = ItemList{all_table, static_cast<int32_t>(all.size()), nullptr};
+  CC_HIP_CHECK(hipMemcpyAsync(all_table, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice, stream_));
+  size_t pi = 0;
   for (auto& [key, list] : pairs) {
-    RebuildGroup& g = rebuild_[gi++];
+    RebuildGroup g{};
     g.wide = key.first;
     g.kc = key.second;
-    g.pairs.count = static_cast<int32_t>(list.size());
-    for (const Item& it : list) g.max_k = std::max(g.max_k, specs_[it.run].params.k);
-    CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(g.pairs.items), list.data(), sizeof(Item) * list.size(),
-                                hipMemcpyHostToDevice, stream_));
+    g.pairs = ItemList{pair_tables[pi], static_cast<int32_t>(list.size()), nullptr};
+    for (const Item& it : list) g.max_k = std::max(g.max_k, spec(it.run).params.k);
+    CC_HIP_CHECK(hipMemcpyAsync(pair_tables[pi], list.data(), sizeof(Item) * list.size(), hipMemcpyHostToDevice,
+                                stream_));
+    rebuild_.push_back(g);
+    ++pi;
   }
   int cus = 0;
-  CC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, specs_[0].graph->device));
+  CC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, spec(0).graph->device));
   for (size_t g = 0; g < groups.size(); ++g) {
-    DescentGroup& d = descent_[g];
-    d.list.count = static_cast<int32_t>(groups[g].second.size());
-    d.list.queue = queues_ + g;
+    DescentGroup d{groups[g].first, ItemList{group_tables[g], static_cast<int32_t>(groups[g].second.size()), queues_ + g}};
     // Persistent workgroups: enough to occupy every CU; a surplus finds the queue empty and exits at once.
     const int per_cu = d.plan.threads >= 1024 ? 1 : (d.plan.threads >= 512 ? 2 : 4);
     d.plan.blocks = std::min(d.list.count, cus * per_cu);
     if (!DescentSupported(d.plan)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
-    CC_HIP_CHECK(hipMemcpyAsync(const_cast<Item*>(d.list.items), groups[g].second.data(),
-                                sizeof(Item) * groups[g].second.size(), hipMemcpyHostToDevice, stream_));
+    CC_HIP_CHECK(hipMemcpyAsync(group_tables[g], groups[g].second.data(), sizeof(Item) * groups[g].second.size(),
+                                hipMemcpyHostToDevice, stream_));
+    descent_.push_back(d);
   }
   CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), sizeof(Ctl) * nruns, hipHostMallocDefault));
   CC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-Batch::~Batch() {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
+LaunchSet::~LaunchSet() {
   if (host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
-void Batch::Rebuild(Timer* timer, cc_pbils_stats* stats) {
+void LaunchSet::Rebuild(Timer* timer, cc_pbils_stats* stats) {
   if (timer) timer->Begin(stream_);
   LaunchMasks(runs_, all_, stream_);
   for (const RebuildGroup& g : rebuild_) LaunchRebuildG(runs_, g, max_n_, stream_);
@@ -409,58 +411,44 @@ void Batch::Rebuild(Timer* timer, cc_pbils_stats* stats) {
   if (timer) timer->End(stream_, &stats->rebuild_ms, &stats->rebuild_launches);
 }
 
-void Batch::Descend(Timer* timer, cc_pbils_stats* stats) {
+void LaunchSet::Descend(Timer* timer, cc_pbils_stats* stats) {
   LaunchResetQueues(queues_, static_cast<int>(descent_.size()), stream_);
   if (timer) timer->Begin(stream_);
   for (const DescentGroup& d : descent_) LaunchDescent(runs_, d.list, d.plan, stream_);
   if (timer) timer->End(stream_, &stats->search_ms, &stats->search_launches);
 }
 
-void Batch::Start() {
-  LaunchResetCtl(runs_, static_cast<int>(specs_.size()), stream_);
+void LaunchSet::Start() {
+  LaunchResetCtl(runs_, Size(), stream_);
   LaunchInitLabels(runs_, all_, max_n_, stream_);
   Rebuild(nullptr, nullptr);
-  LaunchRecord(runs_, static_cast<int>(specs_.size()), kRecordInitial, 0, stream_);
+  LaunchRecord(runs_, Size(), kRecordInitial, 0, stream_);
   CC_HIP_CHECK(hipGetLastError());
 }
 
-void Batch::EnqueueIteration(Timer* timer, cc_pbils_stats* stats) {
+void LaunchSet::EnqueueIteration(Timer* timer, cc_pbils_stats* stats) {
   Descend(timer, stats);
-  LaunchRecord(runs_, static_cast<int>(specs_.size()), kRecordIteration, 1, stream_);
+  LaunchRecord(runs_, Size(), kRecordIteration, 1, stream_);
   LaunchPerturb(runs_, all_, max_n_, stream_);
   Rebuild(timer, stats);
 }
 
-void Batch::Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats) {
-  if (!use_graph || timer) {
-    for (int i = 0; i < count; ++i) EnqueueIteration(timer, stats);
-    CC_HIP_CHECK(hipGetLastError());
-    return;
+void LaunchSet::SetLdsAttributes() {
+  for (const DescentGroup& d : descent_) {
+    const ItemList none{d.list.items, 0, d.list.queue};
+    LaunchDescent(runs_, none, d.plan, stream_);  // no items: only the kernel attributes are set
   }
-  if (!graph_exec_) {
-    // one iteration as a graph: the LDS attributes are set before the capture
-    for (const DescentGroup& d : descent_) {
-      const ItemList none{d.list.items, 0, d.list.queue};
-      LaunchDescent(runs_, none, d.plan, stream_);
-    }
-    CC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    EnqueueIteration(nullptr, nullptr);
-    CC_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
-    CC_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-  }
-  for (int i = 0; i < count; ++i) CC_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
 }
 
-const Ctl* Batch::PullCtl() {
-  CC_HIP_CHECK(hipMemcpyAsync(host_ctl_, host_runs_[0].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
-  for (size_t i = 1; i < host_runs_.size(); ++i) {
+const Ctl* LaunchSet::PullCtl() {
+  for (size_t i = 0; i < host_runs_.size(); ++i) {
     CC_HIP_CHECK(hipMemcpyAsync(host_ctl_ + i, host_runs_[i].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
   }
   CC_HIP_CHECK(hipStreamSynchronize(stream_));
   return host_ctl_;
 }
 
-void Batch::PullLabels(int run, int32_t* labels_out) {
+void LaunchSet::PullLabels(int run, int32_t* labels_out) {
   const RunDev& r = host_runs_[run];
   std::vector<uint8_t> labels(r.n);
   CC_HIP_CHECK(hipMemcpyAsync(labels.data(), r.rec_labels, labels.size(), hipMemcpyDeviceToHost, stream_));
@@ -468,22 +456,60 @@ void Batch::PullLabels(int run, int32_t* labels_out) {
   for (int v = 0; v < r.n; ++v) labels_out[v] = labels[v];
 }
 
-void Batch::PullTrace(int run, int count, int64_t* out) {
+void LaunchSet::PullTrace(int run, int count, int64_t* out) {
   if (count <= 0 || !out) return;
   CC_HIP_CHECK(hipMemcpyAsync(out, host_runs_[run].trace, sizeof(int64_t) * count, hipMemcpyDeviceToHost, stream_));
   CC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 // Neighborhood uses one run whose population is the batch of starts; the last batch may be smaller.
-void Batch::SetPopulation(int run, int population) {
+void LaunchSet::SetPopulation(int run, int population) {
   host_runs_[run].population = population;
   CC_HIP_CHECK(hipMemcpyAsync(runs_ + run, &host_runs_[run], sizeof(RunDev), hipMemcpyHostToDevice, stream_));
 }
 
-ItemList Batch::Prefix(const ItemList& list, int count) const {
+ItemList LaunchSet::Prefix(const ItemList& list, int count) const {
   ItemList l = list;
   l.count = std::min(l.count, count);
   return l;
+}
+
+DescentPlan LaunchSet::Plan(int run) const {
+  for (const DescentGroup& d : descent_) {
+    if (SameLaunch(d.plan, states_[run]->spec().plan)) return d.plan;
+  }
+  return states_[run]->spec().plan;
+}
+
+Batch::Batch(const std::vector<RunSpec>& specs, hipStream_t stream) : stream_(stream) {
+  std::vector<RunState*> raw;
+  for (const RunSpec& s : specs) {
+    states_.push_back(std::make_unique<RunState>(s, stream));
+    raw.push_back(states_.back().get());
+    bytes_ += states_.back()->bytes();
+  }
+  set_ = std::make_unique<LaunchSet>(raw, stream);
+}
+
+Batch::~Batch() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+}
+
+void Batch::Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats) {
+  if (!use_graph || timer) {
+    for (int i = 0; i < count; ++i) set_->EnqueueIteration(timer, stats);
+    CC_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (!graph_exec_) {
+    set_->SetLdsAttributes();  // outside the capture
+    CC_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    set_->EnqueueIteration(nullptr, nullptr);
+    CC_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+    CC_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < count; ++i) CC_HIP_CHECK(hipGraphLaunch(graph_exec_, stream_));
 }
 
 // ---- solves ------------------------------------------------------------------------------------------------------

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -117,45 +118,59 @@ struct DescentGroup {
   ItemList list;
 };
 
-// Device state of a set of independent runs: one allocation, the run table, the item tables of every launch and
-// the persistent-queue counters; an iteration of all runs is one fixed launch sequence (captured once as a graph).
-class Batch {
+// Device state of one run (populations, record, trace and scratch) in one zeroed allocation. It outlives the launch
+// sets it takes part in, so a serving lane can add and drop runs between iterations.
+class RunState {
  public:
-  Batch(std::vector<RunSpec> specs, hipStream_t stream);
-  ~Batch();
-  Batch(const Batch&) = delete;
-  Batch& operator=(const Batch&) = delete;
+  RunState(const RunSpec& spec, hipStream_t stream);
+  RunState(const RunState&) = delete;
+  RunState& operator=(const RunState&) = delete;
+  const RunSpec& spec() const { return spec_; }
+  const RunDev& dev() const { return dev_; }
+  size_t bytes() const { return bytes_; }
+  // Device bytes a run of this spec allocates (the graph excluded).
+  static size_t Footprint(const RunSpec& spec);
+
+ private:
+  RunSpec spec_;
+  RunDev dev_{};
+  DeviceMemory memory_;
+  size_t bytes_ = 0;
+};
+
+// The launches over a set of runs: the device run table, the item tables of every launch (slots, rebuild slot
+// pairs, descent items grouped by kernel instance and ordered largest run first) and the persistent-queue
+// counters. An iteration of all runs is one fixed launch sequence.
+class LaunchSet {
+ public:
+  LaunchSet(const std::vector<RunState*>& states, hipStream_t stream);
+  ~LaunchSet();
+  LaunchSet(const LaunchSet&) = delete;
+  LaunchSet& operator=(const LaunchSet&) = delete;
 
   void Start();  // reset, initial labels, rebuild, initial record
   void EnqueueIteration(Timer* timer, cc_pbils_stats* stats);
-  void Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats);
   void Rebuild(Timer* timer, cc_pbils_stats* stats);
   void Descend(Timer* timer, cc_pbils_stats* stats);
+  void SetLdsAttributes();
 
   const Ctl* PullCtl();
   void PullLabels(int run, int32_t* labels_out);
   void PullTrace(int run, int count, int64_t* out);
   void SetPopulation(int run, int population);
 
+  int Size() const { return static_cast<int>(states_.size()); }
   const RunDev* Runs() const { return runs_; }
   const ItemList& AllItems() const { return all_; }
   const std::vector<RebuildGroup>& RebuildGroups() const { return rebuild_; }
   const std::vector<DescentGroup>& DescentGroups() const { return descent_; }
   ItemList Prefix(const ItemList& list, int count) const;
-  DescentPlan Plan(int run) const {
-    for (const DescentGroup& d : descent_) {
-      if (SameLaunch(d.plan, specs_[run].plan)) return d.plan;
-    }
-    return specs_[run].plan;
-  }
-  size_t Bytes() const { return bytes_; }
-  hipStream_t stream() const { return stream_; }
+  DescentPlan Plan(int run) const;
 
  private:
-  std::vector<RunSpec> specs_;
+  std::vector<RunState*> states_;
   hipStream_t stream_;
   DeviceMemory memory_;
-  size_t bytes_ = 0;
   std::vector<RunDev> host_runs_;
   RunDev* runs_ = nullptr;
   ItemList all_{};
@@ -164,6 +179,35 @@ class Batch {
   int32_t* queues_ = nullptr;
   int max_n_ = 0;
   Ctl* host_ctl_ = nullptr;
+};
+
+// A fixed set of runs with its launch set; iterations are replayed from one captured HIP graph.
+class Batch {
+ public:
+  Batch(const std::vector<RunSpec>& specs, hipStream_t stream);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  void Start() { set_->Start(); }
+  void Iterate(int count, bool use_graph, Timer* timer, cc_pbils_stats* stats);
+  const Ctl* PullCtl() { return set_->PullCtl(); }
+  void PullLabels(int run, int32_t* labels_out) { set_->PullLabels(run, labels_out); }
+  void PullTrace(int run, int count, int64_t* out) { set_->PullTrace(run, count, out); }
+  void SetPopulation(int run, int population) { set_->SetPopulation(run, population); }
+  const RunDev* Runs() const { return set_->Runs(); }
+  const ItemList& AllItems() const { return set_->AllItems(); }
+  const std::vector<RebuildGroup>& RebuildGroups() const { return set_->RebuildGroups(); }
+  const std::vector<DescentGroup>& DescentGroups() const { return set_->DescentGroups(); }
+  ItemList Prefix(const ItemList& list, int count) const { return set_->Prefix(list, count); }
+  DescentPlan Plan(int run) const { return set_->Plan(run); }
+  size_t Bytes() const { return bytes_; }
+
+ private:
+  hipStream_t stream_;
+  std::vector<std::unique_ptr<RunState>> states_;
+  std::unique_ptr<LaunchSet> set_;
+  size_t bytes_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
 };

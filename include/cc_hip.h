@@ -84,6 +84,8 @@ typedef struct cc_pbils_stats {
 #define CC_DESCENT_K2_REGISTERS 2
 #define CC_DESCENT_K2_WIDE 3
 #define CC_DESCENT_K3_WIDE 4
+#define CC_DESCENT_REGISTERS 5 /* k in [3, 8], n <= 32767: state in VGPRs */
+#define CC_DESCENT_MIXED 6     /* batched: one launch over runs of different register kernels */
 
 void cc_pbils_default_params(cc_pbils_params* params);
 
@@ -129,6 +131,37 @@ typedef struct cc_batch_run {
 
 int cc_hip_solve_batch(cc_batch_run* runs, int32_t count, int32_t use_graph, double* seconds, char* err,
                        size_t err_len);
+
+/* ---- serving with continuous batching: requests are queued, admitted between iterations while their HBM
+ *      footprint (n^2/8 bitset + per-vertex state of the population, cc_hip_footprint) fits the budget, advanced
+ *      together with the other active runs of their lane, and retired as soon as they stop. Each result is
+ *      bit-identical to a solo solve. No reference counterpart. ---------------------------------------------- */
+
+#define CC_HIP_ETIMEOUT 5 /* cc_hip_server_wait: not done within the timeout */
+
+typedef struct cc_serve_options {
+  int64_t budget_bytes; /* device bytes the server may hold; <= 0: 85% of the free memory at creation */
+  int32_t lanes;        /* streams with their own continuous batch; lane 0 takes the large requests */
+  int32_t max_active;   /* active runs per lane */
+  int64_t large_bytes;  /* requests with a footprint at or above this go to lane 0 */
+} cc_serve_options;
+
+typedef struct cc_hip_server cc_hip_server;
+
+void cc_serve_default_options(cc_serve_options* options);
+int cc_hip_server_create(const cc_serve_options* options, cc_hip_server** out, char* err, size_t err_len);
+void cc_hip_server_destroy(cc_hip_server* server); /* finishes every queued request first */
+/* Copies the graph; returns at once with the request id. */
+int cc_hip_server_submit(cc_hip_server* server, const uint64_t* bits, uint32_t n, uint64_t edges,
+                         const cc_pbils_params* params, uint64_t* id_out, char* err, size_t err_len);
+/* Blocks until the request is done (timeout_s <= 0: no limit). latency_s: submit to result; queued_s: submit to
+ * admission. trace_out (nullable): params.iterations entries. */
+int cc_hip_server_wait(cc_hip_server* server, uint64_t id, double timeout_s, int32_t* labels_out, int64_t* trace_out,
+                       cc_pbils_stats* stats, double* latency_s, double* queued_s, char* err, size_t err_len);
+int64_t cc_hip_footprint(uint32_t n, const cc_pbils_params* params);
+/* Requests admitted and completed, budget left now, and the peaks of active runs and of admitted bytes. */
+void cc_hip_server_counters(cc_hip_server* server, int64_t* admitted, int64_t* completed, int64_t* budget_left,
+                            int64_t* peak_active, int64_t* peak_bytes);
 
 /* ---- host-side graph generators and recounts of the product (the reference's Graph::ErdosRenyi,
  *      src/cc_graph.cpp:168-177, and the recounts of src/cc_pbils_cpu.cpp:12-53) --------------------------- */
