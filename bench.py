@@ -23,6 +23,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Memory-fabric bytes of the descent kernel, measured: the mean over the launches of a solve of FETCH_SIZE x 2.0 plus
+# WRITE_SIZE x 1.0 from two rocprofv3 --pmc runs (a 1 GiB device copy reads as FETCH_SIZE 0.5 GiB and WRITE_SIZE
+# 1 GiB on this device, hence the factors). Per config: (iterations of the profiled solve, bytes per launch, bytes per
+# accepted move). A timed solve of the same length has the same trajectory, so its launches move the same bytes;
+# other lengths are scaled by their accepted moves. Commands and counter values: profiles/descent_*.txt.
+MEASURED_TRAFFIC = {
+    "c2": (200, 808289686, 808289686 / (101473818 / 200)),
+    "c5": (2, 5251166771568, 813040.0),
+}
+
 METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
 
@@ -40,6 +50,11 @@ CONFIGS = {
 }
 
 
+def note(msg):
+    """A progress line on stderr as each phase of the bench ends (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_graph(cga, gen):
     if gen[0] == "er":
         return cga.erdos_renyi(gen[1], gen[2], gen[3])
@@ -48,13 +63,19 @@ def make_graph(cga, gen):
     return cga.sbm(gen[1], gen[2], gen[3], gen[4], gen[5])
 
 
-def descent_bytes_per_move(n, k, descent):
-    """Algorithmic bytes one accepted move streams: the slot's state scanned once (G and labels, or e and the
-    label bit for the k = 2 register kernel) plus the adjacency row of the moved vertex."""
+def descent_bytes_per_move(n, k, descent, mean_degree):
+    """Algorithmic bytes one accepted move streams: the slot's state scanned once plus the adjacency row of the
+    moved vertex (and, for the packed wide kernels, the write-back of the neighbours' words). The state is G and
+    the labels (k n |G| + n) for the generic and register kernels, e and the label bit (4 n) for the k = 2 register
+    kernel, and one packed word per vertex (4 n for k = 2, 8 n for k = 3) for the wide kernels."""
     words = max(1, (n + 63) // 64)
     row = 8 * words
     if descent == "k2-registers":
         return 4 * n + row
+    if descent == "k2-wide":
+        return 4 * n + row + 4 * mean_degree
+    if descent == "k3-wide":
+        return 8 * n + row + 8 * mean_degree
     gbytes = 2 if n <= 32767 else 4
     return k * n * gbytes + n + row
 
@@ -90,6 +111,7 @@ def cpu_sample(cfg, graph, dg, threads):
         out[f"cpu_{it}"] = cpu["seconds"]
         out[f"gpu_{it}"] = gpu["seconds"]
         out["bit_exact"] &= same_result(cpu, gpu)
+        note(f"cpu sample: {it} iterations, reference {cpu['seconds']:.2f} s")
     out["cpu_iteration_s"] = out[f"cpu_{m + 1}"] - out[f"cpu_{m}"]
     out["gpu_iteration_s"] = out[f"gpu_{m + 1}"] - out[f"gpu_{m}"]
     return out
@@ -101,7 +123,7 @@ def percentile(values, q):
     return float(np.percentile(np.asarray(values, dtype=np.float64), q))
 
 
-def bench_batch(cga, count, population=64, iterations=50, master=7):
+def bench_batch(cga, count, population=64, iterations=50, master=7, plain_leg=True):
     """The mixed stream: `count` independent instances (n log-uniform in [200, 8000], ER or SBM, k in 2..8), pop 64,
     50 iterations, gww 0.1, seeds from master seed 7; solved plain (one instance per solve call, one after the
     other) and batched (one cc_hip_solve_batch), with every batched result compared to its plain result."""
@@ -114,19 +136,25 @@ def bench_batch(cga, count, population=64, iterations=50, master=7):
     cga.solve_batch(graphs[:4], params[:4])  # warm-up: kernels loaded, graph path exercised
     plain, latencies = [], []
     t0 = time.perf_counter()
-    for g, kw in zip(graphs, params):
+    for g, kw in zip(graphs if plain_leg else [], params):
         t1 = time.perf_counter()
         plain.append(g.solve_pbils(**kw))
         latencies.append(time.perf_counter() - t1)
+        if len(plain) % 128 == 0:
+            note(f"mixed stream: {len(plain)}/{count} plain solves")
     plain_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     batched, _ = cga.solve_batch(graphs, params)
     batch_s = time.perf_counter() - t0
+    note(f"mixed stream: batched solve of {count} in {batch_s:.2f} s")
     exact = all(a["objective"] == b["objective"] and a["accepted_moves"] == b["accepted_moves"] and
                 a["iterations_done"] == b["iterations_done"] and bool((a["labels"] == b["labels"]).all())
                 for a, b in zip(plain, batched))
     for g in graphs:
         g.close()
+    if not plain_leg:
+        return {"batched_graphs_per_s": round(count / batch_s, 3), "batched_seconds": round(batch_s, 3),
+                "moves": int(sum(r["accepted_moves"] for r in batched))}
     return {
         "workload": f"{count} instances, n log-uniform [200, 8000], ER p~U[0.05,0.5) or SBM 2-8 blocks "
                     f"(0.4/0.05) 50/50, k~U{{2..8}}, pop {population}, {iterations} iterations, gww 0.1, "
@@ -182,6 +210,7 @@ def bench_serve(cga, rate=50.0, pool=96, big_iterations=20, check=16, master=7):
         feeder = threading.Thread(target=feed)
         feeder.start()
         big_res = server.wait(big_id)
+        note(f"serving: wide run done after {time.perf_counter() - t0:.2f} s, {len(submitted)} requests submitted")
         stop.set()
         feeder.join()
         stream_s = time.perf_counter() - t0
@@ -216,6 +245,20 @@ def bench_serve(cga, rate=50.0, pool=96, big_iterations=20, check=16, master=7):
     }
 
 
+def copy_calibration(mib, reps=4):
+    """Device-to-device copies of `mib` MiB, for scaling hardware byte counters: each copy reads and writes exactly
+    mib MiB of HBM."""
+    import torch
+
+    src = torch.full((mib << 20,), 1, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "copies": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,10 +267,16 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
+    ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson request rate; 0 skips serving")
     ap.add_argument("--serve-iterations", type=int, default=20, help="iterations of the wide-path served run")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (to scale byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
@@ -246,6 +295,7 @@ def main():
         t0 = time.perf_counter()
         res = dg.solve_pbils(iterations=steps, early_stop=steps, profile=True, **common)
         wall = time.perf_counter() - t0
+        note(f"{args.config}: {res['iterations_done']} iterations in {wall:.3f} s")
         sample = None if args.no_cpu else cpu_sample(cfg, graph, dg, args.cpu_threads)
 
     n, k = graph.n, cfg["k"]
@@ -254,7 +304,11 @@ def main():
     launches = max(1, res["search_launches"])
     mean_launch_s = res["search_ms"] / 1e3 / launches
     moves_per_launch = res["accepted_moves"] / launches
-    bytes_per_launch = moves_per_launch * descent_bytes_per_move(n, k, res["descent_name"])
+    bytes_per_launch = moves_per_launch * descent_bytes_per_move(n, k, res["descent_name"], 2.0 * graph.edges / n)
+    traffic = None
+    if args.config in MEASURED_TRAFFIC:
+        t_steps, t_launch, t_move = MEASURED_TRAFFIC[args.config]
+        traffic = t_launch if t_steps == steps else round(t_move * moves_per_launch)
     achieved = bytes_per_launch / mean_launch_s / 1e9 if mean_launch_s > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -284,7 +338,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": round(bytes_per_launch),
             "mean_launch_ms": round(1e3 * mean_launch_s, 4),
         },
@@ -307,7 +361,7 @@ def main():
         }
     else:
         out["cpu_baseline"] = None
-    out["batched"] = bench_batch(cga, args.batch_count) if args.batch_count > 0 else None
+    out["batched"] = bench_batch(cga, args.batch_count, plain_leg=not args.batch_only) if args.batch_count > 0 else None
     out["serving"] = (bench_serve(cga, rate=args.serve_rate, big_iterations=args.serve_iterations)
                       if args.serve_rate > 0 else None)
     print(json.dumps(out))
