@@ -1,5 +1,5 @@
 // cc_internal.hpp — definitions shared by the kernels (cc_kernels.hip, cc_descent.hip) and the host runtime
-// (cc_runtime.hip, cc_batch.hip).
+// (cc_runtime.hip, cc_serve.hip).
 //
 // Every kernel works on a batch: it reads the run descriptors (RunDev) of many independent PBILS runs from a device
 // array and finds its run and slot through an item table. A single solve is a batch of one run.
