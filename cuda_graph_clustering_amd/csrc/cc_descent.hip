@@ -285,28 +285,38 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
 // row words come in with scalar loads.
 using ConstWords = const __attribute__((address_space(4))) uint64_t*;
 
-// k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt. Per vertex:
-//   gs[c] = 128 G[v][c]   and   keya = key of its own cluster,
-// where the key of target c is  key_c = ((2 G_c - s_c + kOff) << 6) | (63 - c) = gs[c] + K_c  with the wave-uniform
-// K_c = ((kOff - s_c) << 6) + 63 - c. The largest key gives the largest 2 G_c - s_c, ties to the smaller c, and
-//   delta(a -> c) = (s_c - s_a + 1) - 2 (G_c - G_a) = (key_a >> 6) - (key_c >> 6) + 1.
-// Including the own cluster a in the maximum is harmless: a move is improving only if its key is strictly above
-// key_a's q by more than one, and then a is not the maximum. A move of v: from -> to changes, for each neighbour u,
-// gs[from] -= 128 and gs[to] += 128; for every vertex with label a it shifts keya by t_a (128 bit - 64) with
-// t_a = [a == to] - [a == from] (G change of a neighbour plus the change of K_a through s_a); t_a is fetched per
-// lane with ds_bpermute from a lane table.
+// k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt.
+//
+// With q_c(v) = s_c - 2 G[v][c], the best move of v goes to the cluster of smallest q_c (ties to the smaller c) and
+// has delta = q_c - q_a + 1 (a = label of v). Per (j, c) one register holds y = -key with
+//   key(j, c) = (q_c << 12) | (j << 6) | (a << 3) | c,
+// and per j the own term qa = q_a << 12. Then z_j = max_c y[j][c] + qa = -x_j with
+//   x_j = ((q_c - q_a) << 12) | (j << 6) | (a << 3) | c,
+// whose signed order is (delta, j, c): the largest z over the thread's vertices is its smallest-delta move, ties to
+// its smaller vertex, and the target comes with it. The scan is thus ceil((k - 1) / 2) v_max3 and one add per
+// vertex. Including c = a in the maximum is harmless: it gives delta 1, never an improving move. The workgroup
+// minimum of ((delta - 1 + 65535) << 15) | v (32 bits, n <= 32767) is the reference's move order (delta, v); the
+// winner's label and target are read from the owner's entry of an LDS table (cand[]).
+//
+// A move v: from -> to changes, with bit = [u is a neighbour of v] and m = 4096 - 8192 bit,
+//   y[from] += m,  y[to] -= m        (s_from - 1, s_to + 1, G[u][from] - bit, G[u][to] + bit)
+//   qa += t_a m,   t_a = [a == to] - [a == from]   (fetched per lane with ds_bpermute from a lane table)
+// for every vertex u; the mover then rewrites its own label bits and qa. Lanes beyond n hold y = -2^30, qa = 0 and
+// a label index (63) whose t is 0: their y drifts by at most 4096 n, so they never offer a move.
 template <int T, int K, int kVpt>
 __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
-  constexpr int kOff = 1 << 16;
-  constexpr int kIdleKey = 1 << 30;  // own key of a vertex beyond n: never improvable
+  constexpr int kPadY = -(1 << 30);
+  constexpr int kPadLabel = 63;
   static_assert(K >= 3 && K <= 8, "register descent handles k in [3, 8]");
+  static_assert(kVpt <= 63, "vertex slot index must fit 6 bits");
   const int n = r.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = WaveIndex();
-  __shared__ uint64_t partial[2][kWaves];
+  __shared__ uint32_t partial[2][kWaves];
+  __shared__ uint8_t cand[2][T];
   __shared__ int32_t src_sh;
   if (tid == 0) {
     src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
@@ -318,27 +328,37 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * K;
 
   int sz[K];
+  int kc[K];  // (s_c << 12) + c
 #pragma unroll
-  for (int c = 0; c < K; ++c) sz[c] = __builtin_amdgcn_readfirstlane(sizes_in[c]);
-  auto kconst = [&](int c) { return ((kOff - sz[c]) << 6) + (63 - c); };
+  for (int c = 0; c < K; ++c) {
+    sz[c] = __builtin_amdgcn_readfirstlane(sizes_in[c]);
+    kc[c] = (sz[c] << 12) + c;
+  }
 
-  int gs[kVpt][K];
-  int lab[kVpt];
-  int keya[kVpt];
+  int y[kVpt][K];
+  int qa[kVpt];
+  int lab4[kVpt];  // label * 4: the ds_bpermute address of the lane table
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
     const int v = tid + j * T;
-    lab[j] = 0;
-    keya[j] = kIdleKey;
+    qa[j] = 0;
+    lab4[j] = kPadLabel * 4;
 #pragma unroll
-    for (int c = 0; c < K; ++c) gs[j][c] = 0;
+    for (int c = 0; c < K; ++c) y[j][c] = kPadY;
     if (v < n) {
-      lab[j] = lab_in[v];
+      const int a = lab_in[v];
+      lab4[j] = a * 4;
+      int ga = 0, sa = 0;
 #pragma unroll
       for (int c = 0; c < K; ++c) {
-        gs[j][c] = static_cast<int>(g_in[static_cast<size_t>(c) * n + v]) << 7;
-        if (lab[j] == c) keya[j] = gs[j][c] + kconst(c);
+        const int g = g_in[static_cast<size_t>(c) * n + v];
+        y[j][c] = (g << 13) - (j << 6) - (a << 3) - kc[c];
+        if (c == a) {
+          ga = g;
+          sa = sz[c];
+        }
       }
+      qa[j] = (sa << 12) - (ga << 13);
     }
   }
   const int64_t f_start = r.pop.f[src];
@@ -346,40 +366,35 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   int64_t applied = 0;
   const int64_t cap = MoveCap(f_start, n);
 
-  int kc[K];
-  auto scan = [&]() {
-#pragma unroll
-    for (int c = 0; c < K; ++c) kc[c] = kconst(c);
-    int best_d = 0, best_j = -1, best_key = 0, best_a = 0;
+  // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
+  auto scan = [&](int parity) {
+    int zmax = kPadY;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      int kb = gs[j][0] + kc[0];
+      int ymax = y[j][0];
 #pragma unroll
-      for (int c = 1; c < K; ++c) kb = max(kb, gs[j][c] + kc[c]);
-      const int d = (keya[j] >> 6) - (kb >> 6) + 1;
-      if (d < best_d) {
-        best_d = d;
-        best_j = j;
-        best_key = kb;
-        best_a = lab[j];
-      }
+      for (int c = 1; c < K; ++c) ymax = max(ymax, y[j][c]);
+      zmax = max(zmax, ymax + qa[j]);
     }
-    if (best_j < 0) return kNoMove;
-    return PackMove(best_d, tid + best_j * T, best_a, 63 - (best_key & 63));
+    const int x = -zmax;  // ((delta - 1) << 12) | (j << 6) | (a << 3) | c
+    cand[parity][tid] = static_cast<uint8_t>(x & 63);
+    if (x >= -4096) return ~0u;  // delta >= 0: no improving move
+    const int j = (x >> 6) & 63;
+    return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(tid + j * T);
   };
 
-  uint64_t key = scan();
   int parity = 0;
+  uint32_t key = scan(parity);
   const ConstWords bits = (ConstWords)r.bits;  // address-space cast (global -> constant)
   while (true) {
-    const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
+    const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
+    if (best == ~0u) break;
+    const int delta = static_cast<int>(best >> 15) - 65535 + 1;
+    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
+    const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
+    const int from = (info >> 3) & 7;
+    const int to = info & 7;
     parity ^= 1;
-    if (best == kNoMove) break;
-    const int delta = static_cast<int>(static_cast<uint32_t>(best >> 32)) - kDeltaOffset;
-    const uint32_t mv = static_cast<uint32_t>(best);
-    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(mv >> 12));
-    const int from = __builtin_amdgcn_readfirstlane(static_cast<int>((mv >> 6) & 63u));
-    const int to = __builtin_amdgcn_readfirstlane(static_cast<int>(mv & 63u));
     const ConstWords row = bits + static_cast<size_t>(v) * r.words;
     uint64_t words[kVpt];
 #pragma unroll
@@ -388,43 +403,49 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       words[j] = base < n ? row[base >> 6] : 0ull;
     }
 #pragma unroll
-    for (int c = 0; c < K; ++c) sz[c] += (c == to) - (c == from);
+    for (int c = 0; c < K; ++c) {
+      sz[c] += (c == to) - (c == from);
+      kc[c] = (sz[c] << 12) + c;
+    }
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
-    int b7[kVpt];
+    int m[kVpt];
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
       const int bit = static_cast<int>((words[j] >> lane) & 1ull);
-      b7[j] = bit << 7;
-      const int ta = __builtin_amdgcn_ds_bpermute(lab[j] << 2, table);
-      keya[j] += ta * (b7[j] - 64);
+      m[j] = 4096 - (bit << 13);
+      const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
+      qa[j] += ta * m[j];
     }
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       if (c == from) {
 #pragma unroll
-        for (int j = 0; j < kVpt; ++j) gs[j][c] -= b7[j];
+        for (int j = 0; j < kVpt; ++j) y[j][c] += m[j];
       }
       if (c == to) {
 #pragma unroll
-        for (int j = 0; j < kVpt; ++j) gs[j][c] += b7[j];
+        for (int j = 0; j < kVpt; ++j) y[j][c] -= m[j];
       }
     }
-    // the moved vertex: new label, own key from its (unchanged) G and the new sizes
+    // the moved vertex: new label bits in its y, its own term from its (unchanged) G[to] and the new s_to
     const int mv_j = v / T;
     if (tid == v - mv_j * T) {
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
         if (j == mv_j) {
-          lab[j] = to;
+          lab4[j] = to * 4;
+          int g_to13 = 0;
 #pragma unroll
           for (int c = 0; c < K; ++c) {
-            if (c == to) keya[j] = gs[j][c] + kconst(c);
+            y[j][c] -= (to - from) << 3;
+            if (c == to) g_to13 = y[j][c] + kc[c] + (j << 6) + (to << 3);
           }
+          qa[j] = (sz[to] << 12) - g_to13;
         }
       }
     }
-    key = scan();
+    key = scan(parity);
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
@@ -441,9 +462,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   for (int j = 0; j < kVpt; ++j) {
     const int v = tid + j * T;
     if (v < n) {
-      lab_out[v] = static_cast<uint8_t>(lab[j]);
+      const int a = lab4[j] >> 2;
+      lab_out[v] = static_cast<uint8_t>(a);
 #pragma unroll
-      for (int c = 0; c < K; ++c) g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>(gs[j][c] >> 7);
+      for (int c = 0; c < K; ++c) {
+        g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>((y[j][c] + kc[c] + (j << 6) + (a << 3)) >> 13);
+      }
     }
   }
   if (tid < K) {
