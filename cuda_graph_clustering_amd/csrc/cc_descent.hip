@@ -89,7 +89,9 @@ __device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
   int parity = 0;
   int prev_from = -1, prev_to = -1;  // the last move, not yet folded into sz[parity]
   int mv_v = -1, mv_from = 0, mv_to = 0;
-  const uint64_t* row = r.bits;
+  const uint64_t* const bits = r.bits;
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  const uint64_t* row = bits;
   while (true) {
     const int32_t* szb = sz[parity];
     int best_d = 0, best_v = -1, best_from = 0, best_to = 0;
@@ -133,7 +135,7 @@ __device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
     prev_from = from;
     prev_to = to;
     parity ^= 1;
-    row = r.bits + static_cast<size_t>(v) * r.words;
+    row = bits + static_cast<size_t>(v) * row_words;
     mv_v = v;
     mv_from = from;
     mv_to = to;
@@ -284,6 +286,8 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
 // Adjacency rows are read-only for the whole run: reading them through the constant address space lets uniform
 // row words come in with scalar loads.
 using ConstWords = const __attribute__((address_space(4))) uint64_t*;
+// The same rows as 32-bit halves in the global address space, for per-lane vector loads.
+using GlobalWords32 = const __attribute__((address_space(1))) uint32_t*;
 
 // k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt.
 //
@@ -385,23 +389,33 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
 
   int parity = 0;
   uint32_t key = scan(parity);
-  const ConstWords bits = (ConstWords)r.bits;  // address-space cast (global -> constant)
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  const uint64_t* const row_base = r.bits;
+  const int half = lane >> 5;
   while (true) {
     const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
+    // The row loads go out before the LDS read of the winner's labels, so the two latencies overlap; every load is
+    // issued (a slot beyond n reads word 0 and discards it), so no branch separates them. They are vector loads
+    // (scalar loads here would be serialised by the kernel's SGPR pressure) of one 32-bit half word per lane: lanes
+    // 0-31 take the low half of the slot's 64-bit row word, lanes 32-63 the high half. Slot j is T / 32 half words
+    // after slot j - 1, so all loads share one address register; slots beyond n read into the zero tail after the
+    // last row (kRowTailBytes) or the next row, and are discarded.
+    const GlobalWords32 row =
+        (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + (wave * kWave >> 5) + half;
+    uint32_t words[kVpt];
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const int base = wave * kWave + j * T;  // wave-uniform
+      const uint32_t w = row[j * (T / 32)];
+      words[j] = base < n ? w : 0u;
+    }
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
     const int from = (info >> 3) & 7;
     const int to = info & 7;
     parity ^= 1;
-    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
-    uint64_t words[kVpt];
-#pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      const int base = wave * kWave + j * T;  // wave-uniform
-      words[j] = base < n ? row[base >> 6] : 0ull;
-    }
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       sz[c] += (c == to) - (c == from);
@@ -412,7 +426,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     int m[kVpt];
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int bit = static_cast<int>((words[j] >> lane) & 1ull);
+      const int bit = static_cast<int>((words[j] >> (lane & 31)) & 1u);
       m[j] = 4096 - (bit << 13);
       const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
       qa[j] += ta * m[j];
@@ -532,6 +546,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   const int64_t cap = MoveCap(f_start, n);
   int parity = 0;
   const ConstWords bits = (ConstWords)r.bits;
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -544,7 +559,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
     sv = n - 2 * n0 + 1;
     const int shift = from ? -4 : 4;
     if (v % T == tid) state[v] ^= 1;
-    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
+    const ConstWords row = bits + static_cast<size_t>(v) * row_words;
     best_d = 0;
     best_u = -1;
     for (int base0 = wave * kWave; base0 < n; base0 += T * kWideUnroll) {
@@ -670,6 +685,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   const int64_t cap = MoveCap(f_start, n);
   int parity = 0;
   const ConstWords bits = (ConstWords)r.bits;
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -686,7 +702,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
     if (v % T == tid) {
       state[v] = (state[v] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
     }
-    const ConstWords row = bits + static_cast<size_t>(v) * r.words;
+    const ConstWords row = bits + static_cast<size_t>(v) * row_words;
     best_d = 0;
     best_move = -1;
     // kWideUnroll vertices per thread in flight: all loads of a group are issued before any is consumed.

@@ -68,16 +68,17 @@ void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hi
   g.words = Words64(n);
   g.edges = edges;
   const size_t count = static_cast<size_t>(n) * g.words;
-  const size_t bytes = count * 8 + static_cast<size_t>(n) * 4 + sizeof(RunDev);
+  const size_t bytes = count * 8 + kRowTailBytes + static_cast<size_t>(n) * 4 + sizeof(RunDev);
   const size_t free_bytes = FreeDeviceBytes();
   if (bytes > free_bytes) {
     Fail(CC_HIP_ENOMEM, "graph needs %s of device memory, %s free", Mebibytes(bytes).c_str(),
          Mebibytes(free_bytes).c_str());
   }
-  g.bits = static_cast<uint64_t*>(g.memory.AllocateBytes(count * sizeof(uint64_t)));
+  g.bits = static_cast<uint64_t*>(g.memory.AllocateBytes(count * sizeof(uint64_t) + kRowTailBytes));
   g.deg = static_cast<int32_t*>(g.memory.AllocateBytes(static_cast<size_t>(n) * sizeof(int32_t)));
   RunDev* probe = static_cast<RunDev*>(g.memory.AllocateBytes(sizeof(RunDev)));
   CC_HIP_CHECK(hipMemcpyAsync(g.bits, bits, count * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+  CC_HIP_CHECK(hipMemsetAsync(g.bits + count, 0, kRowTailBytes, stream));
   RunDev r{};
   r.bits = g.bits;
   r.deg = g.deg;

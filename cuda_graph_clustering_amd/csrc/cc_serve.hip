@@ -54,7 +54,9 @@ struct Active {
   std::unique_ptr<RunState> state;
 };
 
-size_t GraphBytes(uint32_t n) { return static_cast<size_t>(n) * Words64(n) * 8 + static_cast<size_t>(n) * 4; }
+size_t GraphBytes(uint32_t n) {
+  return static_cast<size_t>(n) * Words64(n) * 8 + kRowTailBytes + static_cast<size_t>(n) * 4;
+}
 
 }  // namespace
 
