@@ -365,7 +365,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       qa[j] = (sa << 12) - (ga << 13);
     }
   }
-  const int64_t f_start = r.pop.f[src];
+  const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;
   int64_t applied = 0;
   const int64_t cap = MoveCap(f_start, n);
@@ -390,7 +390,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   int parity = 0;
   uint32_t key = scan(parity);
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
-  const uint64_t* const row_base = r.bits;
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int half = lane >> 5;
   while (true) {
     const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
@@ -417,8 +418,9 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     const int to = info & 7;
     parity ^= 1;
 #pragma unroll
-    for (int c = 0; c < K; ++c) {
-      sz[c] += (c == to) - (c == from);
+    for (int c = 0; c < K; ++c) {  // uniform: scalar branches
+      if (c == to) ++sz[c];
+      if (c == from) --sz[c];
       kc[c] = (sz[c] << 12) + c;
     }
     // lane c of the table holds t_c = [c == to] - [c == from]
@@ -426,10 +428,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     int m[kVpt];
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int bit = static_cast<int>((words[j] >> (lane & 31)) & 1u);
-      m[j] = 4096 - (bit << 13);
+      // m = 4096 - 8192 bit, from the sign-extended one-bit field (0 or -1)
+      m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(words[j]), lane & 31, 1) << 13) + 4096;
       const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
-      qa[j] += ta * m[j];
+      qa[j] += __mul24(ta, m[j]);  // |ta| <= 1, |m| = 4096: a full-rate 24-bit multiply
     }
 #pragma unroll
     for (int c = 0; c < K; ++c) {

@@ -89,6 +89,13 @@ __device__ __forceinline__ T BlockMin(T key, T* partial) {
   }
 }
 
+// A value the same in every lane, moved into scalar registers.
+__device__ __forceinline__ int64_t Uniform(int64_t x) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32)));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // Wave-uniform index of the calling wave.
 __device__ __forceinline__ int WaveIndex() {
   return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
