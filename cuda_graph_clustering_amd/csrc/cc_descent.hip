@@ -498,8 +498,57 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   }
 }
 
-// Vertices per thread whose loads a wide-path scan keeps in flight at once.
+// Vertices per thread whose loads a wide-path scan keeps in flight per group.
 constexpr int kWideUnroll = 8;
+
+template <typename Word>
+using GlobalPtr = __attribute__((address_space(1))) Word*;
+
+// One pass of a wide-path descent over a slot's packed vertex words after a move: the word of every neighbour of
+// the moved vertex gets `inc` and is written back, then visit(w, u) sees every word u < n of the thread's share.
+// Words are loaded in groups of kWideUnroll per thread and group g + 1 is loaded before group g is consumed, so two
+// groups of loads are in flight behind the arithmetic. The row of the moved vertex is read as one 32-bit half word
+// per lane (lanes 0-31 the low half of the 64-bit word, 32-63 the high half); a group past n reads, clamped, at
+// most one half word beyond the row (the next row or the zero tail after the last row) and discards it.
+template <int T, typename Word, typename Visit>
+__device__ __forceinline__ void StreamMove(GlobalPtr<Word> st, GlobalWords32 row, int n, int wave, int lane, Word inc,
+                                           Visit&& visit) {
+  constexpr int kU = kWideUnroll;
+  const int half = lane >> 5;
+  const int bitpos = lane & 31;
+  Word cw[kU], nw[kU];
+  uint32_t cr[kU], nr[kU];
+  auto load = [&](int base0, Word* w, uint32_t* rr) {
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int base = base0 + i * T;  // wave-uniform
+      const int u = base + lane;
+      rr[i] = row[((base < n ? base : n - 1) >> 5) + half];
+      w[i] = u < n ? st[u] : Word(0);
+    }
+  };
+  load(wave * kWave, cw, cr);
+  for (int base0 = wave * kWave; base0 < n; base0 += T * kU) {
+    load(base0 + T * kU, nw, nr);
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int u = base0 + i * T + lane;
+      if (u < n) {
+        Word w = cw[i];
+        if ((cr[i] >> bitpos) & 1u) {
+          w += inc;
+          st[u] = w;
+        }
+        visit(w, u);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      cw[i] = nw[i];
+      cr[i] = nr[i];
+    }
+  }
+}
 
 // Wide path, k = 2: one 32-bit word per vertex, w = 2 e + label with e = G1 - G0, kept in the slot's row G[0]
 // during the descent (reference algebra cuda/cc_local_search_k2.cuh:251-254): delta = sv - w for label 0 and
@@ -547,8 +596,10 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   int64_t applied = 0;
   const int64_t cap = MoveCap(f_start, n);
   int parity = 0;
-  const ConstWords bits = (ConstWords)r.bits;
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  const GlobalPtr<int32_t> st = (GlobalPtr<int32_t>)state;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -560,38 +611,18 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
     n0 += from ? 1 : -1;
     sv = n - 2 * n0 + 1;
     const int shift = from ? -4 : 4;
-    if (v % T == tid) state[v] ^= 1;
-    const ConstWords row = bits + static_cast<size_t>(v) * row_words;
+    if (v % T == tid) st[v] ^= 1;
+    const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_u = -1;
-    for (int base0 = wave * kWave; base0 < n; base0 += T * kWideUnroll) {
-      uint64_t words[kWideUnroll];
-      int ws[kWideUnroll];
-#pragma unroll
-      for (int i = 0; i < kWideUnroll; ++i) {
-        const int base = base0 + i * T;
-        const int u = base + lane;
-        words[i] = base < n ? row[base >> 6] : 0ull;
-        ws[i] = u < n ? state[u] : 0;
+    StreamMove<T, int32_t>(st, row, n, wave, lane, shift, [&](int w, int u) {
+      const int d = (w & 1) ? w + 1 - sv : sv - w;
+      if (d < best_d) {
+        best_d = d;
+        best_u = u;
+        best_w = w;
       }
-#pragma unroll
-      for (int i = 0; i < kWideUnroll; ++i) {
-        const int u = base0 + i * T + lane;
-        if (u < n) {
-          int w = ws[i];
-          if ((words[i] >> lane) & 1ull) {
-            w += shift;
-            state[u] = w;
-          }
-          const int d = (w & 1) ? w + 1 - sv : sv - w;
-          if (d < best_d) {
-            best_d = d;
-            best_u = u;
-            best_w = w;
-          }
-        }
-      }
-    }
+    });
     key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
     objective += delta;
     if (++applied >= cap) {
@@ -686,8 +717,10 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   int64_t applied = 0;
   const int64_t cap = MoveCap(f_start, n);
   int parity = 0;
-  const ConstWords bits = (ConstWords)r.bits;
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)state;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -702,35 +735,12 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
     s2 += (to == 2) - (from == 2);
     const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
     if (v % T == tid) {
-      state[v] = (state[v] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
+      st[v] = (st[v] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
     }
-    const ConstWords row = bits + static_cast<size_t>(v) * row_words;
+    const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_move = -1;
-    // kWideUnroll vertices per thread in flight: all loads of a group are issued before any is consumed.
-    for (int base0 = wave * kWave; base0 < n; base0 += T * kWideUnroll) {
-      uint64_t words[kWideUnroll];
-      uint64_t ws[kWideUnroll];
-#pragma unroll
-      for (int i = 0; i < kWideUnroll; ++i) {
-        const int base = base0 + i * T;
-        const int u = base + lane;
-        words[i] = base < n ? row[base >> 6] : 0ull;
-        ws[i] = u < n ? state[u] : 0ull;
-      }
-#pragma unroll
-      for (int i = 0; i < kWideUnroll; ++i) {
-        const int u = base0 + i * T + lane;
-        if (u < n) {
-          uint64_t w = ws[i];
-          if ((words[i] >> lane) & 1ull) {
-            w += inc;
-            state[u] = w;
-          }
-          consider(w, u);
-        }
-      }
-    }
+    StreamMove<T, uint64_t>(st, row, n, wave, lane, inc, consider);
     key = key_of();
     objective += delta;
     if (++applied >= cap) {
