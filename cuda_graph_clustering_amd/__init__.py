@@ -93,6 +93,12 @@ class ServeOptions(ctypes.Structure):
     ]
 
 
+class TuneCandidate(ctypes.Structure):
+    _fields_ = [("descent", ctypes.c_int32), ("threads", ctypes.c_int32), ("vertices_per_thread", ctypes.c_int32),
+                ("rows_in_lds", ctypes.c_int32), ("blocks", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_int64), ("accepted_moves", ctypes.c_int64), ("search_ms", ctypes.c_double)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 64),
@@ -134,6 +140,10 @@ ABI_SYMBOLS = [
     "cc_objective_direct",
     "cc_objective_bitwise",
     "cc_hip_device_info",
+    "cc_hip_autotune",
+    "cc_hip_tune_clear",
+    "cc_hip_tune_save",
+    "cc_hip_tune_load",
 ]
 
 
@@ -183,6 +193,13 @@ def _load():
         getattr(lib, name).argtypes = [_u64p, ctypes.c_uint32, _i32p]
         getattr(lib, name).restype = ctypes.c_int64
     lib.cc_hip_device_info.argtypes = [ctypes.POINTER(DeviceInfo), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32,
+                                    ctypes.POINTER(TuneCandidate), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_tune_clear.argtypes = []
+    lib.cc_hip_tune_save.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_tune_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p,
+                                     ctypes.c_size_t]
     return lib
 
 
@@ -360,6 +377,46 @@ def solve_batch(device_graphs, params, use_graph=True):
     _check(rc, err)
     results = [_result(runs[i].stats, keep[i][0], keep[i][2], keep[i][1]) for i in range(count)]
     return results, seconds.value
+
+
+def autotune(device_graph, tune_iterations=2, **kw):
+    """Times every descent plan for this graph's shape on `tune_iterations` PBILS iterations (cc_hip_autotune) and
+    records the fastest for ls_kernel = auto. Returns (candidates, chosen): one dict per plan, with its device time."""
+    p = make_params(**kw)
+    capacity = 64
+    out = (TuneCandidate * capacity)()
+    count = ctypes.c_int32(0)
+    chosen = ctypes.c_int32(-1)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.cc_hip_autotune(device_graph.handle, ctypes.byref(p), tune_iterations, out, capacity, ctypes.byref(count),
+                             ctypes.byref(chosen), err, 512)
+    _check(rc, err)
+    cands = []
+    for i in range(min(count.value, capacity)):
+        c = out[i]
+        cands.append({"descent": DESCENTS.get(c.descent, str(c.descent)), "threads": c.threads,
+                      "vertices_per_thread": c.vertices_per_thread, "rows_in_lds": bool(c.rows_in_lds),
+                      "blocks": c.blocks, "lds_bytes": c.lds_bytes, "accepted_moves": c.accepted_moves,
+                      "search_ms": c.search_ms})
+    return cands, chosen.value
+
+
+def tune_clear():
+    """Forget every tuned plan."""
+    lib.cc_hip_tune_clear()
+
+
+def tune_save(path):
+    err = ctypes.create_string_buffer(512)
+    _check(lib.cc_hip_tune_save(os.fsencode(path), err, 512), err)
+
+
+def tune_load(path):
+    """Adds the plans of a saved table; returns how many were read."""
+    entries = ctypes.c_int32(0)
+    err = ctypes.create_string_buffer(512)
+    _check(lib.cc_hip_tune_load(os.fsencode(path), ctypes.byref(entries), err, 512), err)
+    return entries.value
 
 
 def footprint(n, **kw):

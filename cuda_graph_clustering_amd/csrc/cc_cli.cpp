@@ -41,6 +41,7 @@ struct Options {
   cc::PbilsParams params;
   int algorithm = kPbils;
   int runs = 1;
+  int autotune = 0;  // iterations per plan; 0: no tuning
   bool verify = true;
   std::string out_path, labels_out_path;
 };
@@ -122,6 +123,8 @@ std::vector<Flag> Flags(Options& o) {
               {"global", cc::PbilsParams::kLocalSearchGlobal},
               {"wide", cc::PbilsParams::kLocalSearchWide}}),
       Option("--runs", "R", "independent runs (seed, seed+1, ...), prints min/avg/max", o.runs),
+      Option("--autotune", "I", "before the runs, time every descent plan on I iterations and use the fastest",
+             o.autotune),
       Switch("--no-verify", "skip the recount of the objective", o.verify, false),
       Option("--out", "PATH", "save the result as JSON", o.out_path),
       Option("--labels-out", "PATH", "save the best clustering as labels", o.labels_out_path),
@@ -238,6 +241,7 @@ int main(int argc, char** argv) {
     std::printf("algorithm    %s  k=%d  batch=%d\n", AlgorithmName(o.algorithm), p.k, p.population);
   }
 
+  if (o.autotune > 0 && o.algorithm == kPbils) std::fputs(cc::AutotuneGpu(graph, p, o.autotune).c_str(), stdout);
   const bool pairwise = graph.Size() <= kPairwiseVerifyLimit;
   cc::PbilsResult best;
   double objective_sum = 0.0, seconds_sum = 0.0;

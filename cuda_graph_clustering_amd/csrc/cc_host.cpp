@@ -416,6 +416,34 @@ PbilsResult SolveNeighborhoodGpu(const Graph& graph, const PbilsParams& params) 
   return Neighborhood(graph, params, false);
 }
 
+std::string AutotuneGpu(const Graph& graph, const PbilsParams& params, int iterations) {
+  static const char* const kNames[] = {"generic, global", "generic, LDS", "k = 2, registers", "k = 2, packed",
+                                       "k = 3, packed",   "registers",    "mixed"};
+  const cc_pbils_params a = ToAbi(params);
+  char err[512] = {0};
+  cc_hip_graph* g = nullptr;
+  if (cc_hip_graph_upload(graph.Bits().data(), graph.Size(), graph.EdgeCount(), &g, err, sizeof(err)) != CC_HIP_OK) {
+    throw std::runtime_error(err);
+  }
+  std::vector<cc_tune_candidate> cands(64);
+  int32_t count = 0, chosen = -1;
+  const int rc = cc_hip_autotune(g, &a, iterations, cands.data(), static_cast<int32_t>(cands.size()), &count, &chosen,
+                                 err, sizeof(err));
+  cc_hip_graph_free(g);
+  if (rc != CC_HIP_OK) throw std::runtime_error(err);
+  std::string report;
+  char line[256];
+  for (int i = 0; i < std::min<int>(count, static_cast<int>(cands.size())); ++i) {
+    const cc_tune_candidate& c = cands[i];
+    const char* name = c.descent >= 0 && c.descent <= 6 ? kNames[c.descent] : "?";
+    std::snprintf(line, sizeof(line), "tune %c %-18s threads=%-5d vpt=%-3d blocks=%-4d lds=%-7lld %10.3f ms\n",
+                  i == chosen ? '*' : ' ', name, c.threads, c.vertices_per_thread, c.blocks,
+                  static_cast<long long>(c.lds_bytes), c.search_ms);
+    report += line;
+  }
+  return report;
+}
+
 PbilsResult SolveNeighborhoodWithManyLocalSearchesGpu(const Graph& graph, const PbilsParams& params) {
   return Neighborhood(graph, params, true);
 }

@@ -119,6 +119,7 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
   const int n = static_cast<int>(n_);
   const int wide = WidePath(n_, p) ? 1 : 0;
   DescentPlan plan;
+  if (p.ls_kernel == CC_LS_AUTO && TunedPlan(n_, k, p.population, wide != 0, &plan)) return plan;
   plan.k = k;
   plan.wide = wide;
   plan.threads = GenericThreads(n);
@@ -388,9 +389,10 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream) :
   CC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, spec(0).graph->device));
   for (size_t g = 0; g < groups.size(); ++g) {
     DescentGroup d{groups[g].first, ItemList{group_tables[g], static_cast<int32_t>(groups[g].second.size()), queues_ + g}};
-    // Persistent workgroups: enough to occupy every CU; a surplus finds the queue empty and exits at once.
+    // Persistent workgroups: enough to occupy every CU (a surplus finds the queue empty and exits at once), unless
+    // the plan names a resident count (the autotuner may keep fewer individuals' state in cache at a time).
     const int per_cu = d.plan.threads >= 1024 ? 1 : (d.plan.threads >= 512 ? 2 : 4);
-    d.plan.blocks = std::min(d.list.count, cus * per_cu);
+    d.plan.blocks = std::min(d.list.count, d.plan.blocks > 0 ? d.plan.blocks : cus * per_cu);
     if (!DescentSupported(d.plan)) Fail(CC_HIP_EINVAL, "no descent kernel for this plan");
     CC_HIP_CHECK(hipMemcpyAsync(group_tables[g], groups[g].second.data(), sizeof(Item) * groups[g].second.size(),
                                 hipMemcpyHostToDevice, stream_));

@@ -62,6 +62,8 @@ void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hi
 void CheckParams(const cc_pbils_params& p);
 bool WidePath(uint32_t n, const cc_pbils_params& p);
 DescentPlan PlanDescent(uint32_t n, int k, const cc_pbils_params& p);
+// The autotuner's recorded plan for a shape, if any (cc_tune.hip).
+bool TunedPlan(uint32_t n, int k, int population, bool wide, DescentPlan* plan);
 
 using Clock = std::chrono::steady_clock;
 double Since(Clock::time_point t0);

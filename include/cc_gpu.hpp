@@ -20,4 +20,9 @@ PbilsResult SolveGpu(const Graph& graph, const PbilsParams& params);
 PbilsResult SolveNeighborhoodGpu(const Graph& graph, const PbilsParams& params);
 PbilsResult SolveNeighborhoodWithManyLocalSearchesGpu(const Graph& graph, const PbilsParams& params);
 
+// Times every descent plan for this graph's shape on `iterations` PBILS iterations and records the fastest for
+// ls_kernel = auto in this process (no counterpart in the reference, which fixes its tuning at compile time).
+// Returns one printable line per plan tried, the chosen one marked.
+std::string AutotuneGpu(const Graph& graph, const PbilsParams& params, int iterations);
+
 }  // namespace cc

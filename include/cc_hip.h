@@ -163,6 +163,32 @@ int64_t cc_hip_footprint(uint32_t n, const cc_pbils_params* params);
 void cc_hip_server_counters(cc_hip_server* server, int64_t* admitted, int64_t* completed, int64_t* budget_left,
                             int64_t* peak_active, int64_t* peak_bytes);
 
+/* ---- descent autotuner: times every descent plan the kernels support for a graph's shape (n, k, population, G
+ *      width) on a short PBILS run of that graph, checks that all follow one trajectory, and records the fastest
+ *      in a process-wide table that ls_kernel = CC_LS_AUTO then uses for that shape. Replaces the reference's
+ *      compile-time tuning for one GPU (cuda/cc_gpu_tuning.cuh:13-77, plan choice cuda/cc_pbils_gpu.cu:69-102). -- */
+
+typedef struct cc_tune_candidate {
+  int32_t descent; /* CC_DESCENT_* */
+  int32_t threads; /* threads per workgroup */
+  int32_t vertices_per_thread;
+  int32_t rows_in_lds; /* k = 2 register kernel: adjacency staged in LDS */
+  int32_t blocks;      /* persistent workgroups launched */
+  int32_t reserved;
+  int64_t lds_bytes; /* dynamic LDS (generic kernel: G and labels in LDS) */
+  int64_t accepted_moves;
+  double search_ms; /* device time of the descent launches */
+} cc_tune_candidate;
+
+/* iterations: PBILS iterations timed per candidate (>= 1). out receives up to capacity candidates; count the number
+ * tried; chosen the index of the fastest (now recorded for this shape). */
+int cc_hip_autotune(cc_hip_graph* graph, const cc_pbils_params* params, int32_t iterations, cc_tune_candidate* out,
+                    int32_t capacity, int32_t* count, int32_t* chosen, char* err, size_t err_len);
+void cc_hip_tune_clear(void);
+/* Text table, one line per shape: n k population wide descent threads vpt rows_in_lds lds_bytes blocks. */
+int cc_hip_tune_save(const char* path, char* err, size_t err_len);
+int cc_hip_tune_load(const char* path, int32_t* entries, char* err, size_t err_len);
+
 /* ---- host-side graph generators and recounts of the product (the reference's Graph::ErdosRenyi,
  *      src/cc_graph.cpp:168-177, and the recounts of src/cc_pbils_cpu.cpp:12-53) --------------------------- */
 

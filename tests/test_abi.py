@@ -79,3 +79,22 @@ def test_cli_binary_is_built():
     import cuda_graph_clustering_amd as cga
 
     assert os.access(cga.CLI_PATH, os.X_OK)
+
+
+def test_tune_table_load_validates_entries(tmp_path):
+    import cuda_graph_clustering_amd as cga
+
+    good = tmp_path / "good.txt"
+    good.write_text("2000 5 16 0 5 1024 2 0 0 0\n")
+    bad = tmp_path / "bad.txt"
+    bad.write_text("2000 5 16 0 99 1024 2 0 0 0\n")
+    garbled = tmp_path / "garbled.txt"
+    garbled.write_text("2000 5 16 0 5 1024 two\n")
+    try:
+        assert cga.tune_load(good) == 1
+        for path in (bad, garbled):
+            with pytest.raises(cga.CcError) as e:
+                cga.tune_load(path)
+            assert e.value.code == cga.EINVAL
+    finally:
+        cga.tune_clear()

@@ -1,0 +1,82 @@
+"""The cc_gpu command line and the instance formats around the path: a solve through the CLI matches the oracle;
+the matrix text, baseline JSON and edge-list files it writes load back to the same graph and the same result; the
+--out JSON keys and --labels-out format are the reference's (include/cc_cli.hpp:248-277, 372-376); invalid input
+ends with `error:` and exit status 1 (include/cc_cli.hpp:329-335)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "cuda_graph_clustering_amd", "bin", "cc_gpu")
+
+PBILS = ["--k", "3", "--pop", "16", "--iters", "5", "--early-stop", "5", "--seed", "7", "--gww", "0.1"]
+
+
+@pytest.fixture(scope="module")
+def cga():
+    import cuda_graph_clustering_amd as m
+
+    return m
+
+
+def run(args, tmp_path, expect_ok=True):
+    proc = subprocess.run([CLI] + [str(a) for a in args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    if expect_ok:
+        assert proc.returncode == 0, proc.stdout + proc.stderr
+    return proc
+
+
+def objective_line(stdout):
+    for line in stdout.splitlines():
+        if line.startswith("run 0"):
+            return int(line.split("f=")[1].split()[0])
+    raise AssertionError(stdout)
+
+
+def test_pbils_through_cli_matches_oracle_and_formats_round_trip(cga, orc, cpu_threads, tmp_path):
+    import oracle
+
+    g = cga.erdos_renyi(300, 0.3, 5)
+    want = orc.solve_pbils(oracle.Graph(g.bits, g.n, g.edges), threads=cpu_threads, k=3, population=16,
+                           iterations=5, early_stop=5, seed=7, gww=0.1)
+    run(["--n", 300, "--density", 0.3, "--graph-seed", 5, *PBILS, "--save-graph", "a.txt", "--save-graph-json",
+         "a.json", "--save-edges", "a.edges", "--out", "r.json", "--labels-out", "l.txt"], tmp_path)
+    result = json.loads((tmp_path / "r.json").read_text())
+    assert result["objective function value"] == want["objective"]
+    assert result["size"] == 300 and result["edges"] == g.edges and result["k"] == 3
+    assert result["clustering vector"] == want["labels"].tolist()
+    labels = np.array([int(x) for x in (tmp_path / "l.txt").read_text().split()])
+    assert (labels == want["labels"]).all()
+    for flag, path in (("--graph", "a.txt"), ("--graph-json", "a.json"), ("--edges", "a.edges")):
+        out = run([flag, path, *PBILS], tmp_path).stdout
+        assert objective_line(out) == want["objective"], flag
+
+
+def test_neighborhood_algorithms_through_cli(cga, orc, tmp_path):
+    import oracle
+
+    g = cga.erdos_renyi(120, 0.25, 3)
+    og = oracle.Graph(g.bits, g.n, g.edges)
+    for algorithm, local_search in (("NeighborhoodCuda", 0), ("NeighborhoodWithManyLocalSearchesCuda", 1)):
+        out = run(["--algorithm", algorithm, "--n", 120, "--density", 0.25, "--graph-seed", 3, "--k", 2, "--pop", 32],
+                  tmp_path).stdout
+        want = orc.neighborhood(og, local_search=local_search)
+        assert objective_line(out) == want["objective"], algorithm
+
+
+def test_autotune_flag_reports_plans(tmp_path):
+    out = run(["--n", 400, "--density", 0.3, "--graph-seed", 2, *PBILS, "--autotune", 1], tmp_path).stdout
+    tuned = [line for line in out.splitlines() if line.startswith("tune")]
+    assert len(tuned) >= 3 and sum(line.startswith("tune *") for line in tuned) == 1
+
+
+def test_invalid_input_is_an_error(tmp_path):
+    proc = run(["--n", 50, "--k", 65], tmp_path, expect_ok=False)
+    assert proc.returncode == 1 and "error:" in proc.stdout
+    proc = run(["--graph", "missing.txt"], tmp_path, expect_ok=False)
+    assert proc.returncode == 1 and "error:" in proc.stdout

@@ -1,0 +1,75 @@
+"""The descent autotuner: every candidate plan follows the same trajectory (the tuner itself refuses a plan that
+leaves it), the chosen plan is the fastest measured, ls_kernel = auto then runs it, and a solve on the tuned plan
+still matches the oracle bit for bit. The table survives a save / clear / load round trip."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("objective", "iterations_done", "clusters_used", "local_searches", "accepted_moves")
+
+
+@pytest.fixture(scope="module")
+def cga():
+    import cuda_graph_clustering_amd as m
+
+    return m
+
+
+@pytest.fixture(autouse=True)
+def fresh_table(cga):
+    cga.tune_clear()
+    yield
+    cga.tune_clear()
+
+
+def check_oracle(cga, orc, g, res, cpu_threads, **kw):
+    import oracle
+
+    want = orc.solve_pbils(oracle.Graph(g.bits, g.n, g.edges), threads=cpu_threads, trace=True, **kw)
+    for key in KEYS:
+        assert res[key] == want[key], key
+    assert (res["labels"] == want["labels"]).all()
+    assert (res["trace"] == want["trace"]).all()
+
+
+@pytest.mark.parametrize("shape", [("sbm", 2000, 4), ("er", 700, 2), ("er", 300, 12)], ids=lambda s: f"{s[0]}{s[1]}k{s[2]}")
+def test_candidates_agree_and_auto_uses_the_fastest(cga, orc, cpu_threads, shape):
+    kind, n, k = shape
+    g = cga.sbm(n, 4, 0.4, 0.05, 3) if kind == "sbm" else cga.erdos_renyi(n, 0.3, 3)
+    kw = dict(k=k, population=12, iterations=6, early_stop=6, gww=0.1, seed=9)
+    with cga.DeviceGraph(g) as dg:
+        cands, chosen = cga.autotune(dg, tune_iterations=2, **kw)
+        assert len(cands) >= 3 and 0 <= chosen < len(cands)
+        assert len({c["accepted_moves"] for c in cands}) == 1
+        assert cands[chosen]["search_ms"] == min(c["search_ms"] for c in cands)
+        res = dg.solve_pbils(**kw)
+    best = cands[chosen]
+    assert (res["descent_name"], res["threads"]) == (best["descent"], best["threads"])
+    check_oracle(cga, orc, g, res, cpu_threads, **kw)
+
+
+def test_wide_shape_tries_resident_counts(cga, orc, cpu_threads):
+    g = cga.chung_lu(40000, 2.5, 16.0, 9)
+    kw = dict(k=3, population=32, iterations=2, early_stop=2, seed=5)
+    with cga.DeviceGraph(g) as dg:
+        cands, chosen = cga.autotune(dg, tune_iterations=1, **kw)
+        assert {c["descent"] for c in cands} >= {"k3-wide", "generic-global"}
+        assert len({c["blocks"] for c in cands if c["descent"] == "k3-wide"}) >= 2
+        res = dg.solve_pbils(**kw)
+    assert res["descent_name"] == cands[chosen]["descent"]
+    check_oracle(cga, orc, g, res, cpu_threads, **kw)
+
+
+def test_table_round_trip(cga, tmp_path):
+    g = cga.erdos_renyi(500, 0.3, 4)
+    kw = dict(k=5, population=8, iterations=3, early_stop=3, seed=2)
+    with cga.DeviceGraph(g) as dg:
+        cands, chosen = cga.autotune(dg, tune_iterations=1, **kw)
+        tuned = dg.solve_pbils(**kw)
+        path = tmp_path / "tune.txt"
+        cga.tune_save(path)
+        cga.tune_clear()
+        assert cga.tune_load(path) == 1
+        again = dg.solve_pbils(**kw)
+    assert (again["descent_name"], again["threads"]) == (tuned["descent_name"], tuned["threads"])
+    assert again["objective"] == tuned["objective"] and (again["labels"] == tuned["labels"]).all()
