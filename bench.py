@@ -259,18 +259,68 @@ def copy_calibration(mib, reps=4):
     print(json.dumps({"copy_calibration_mib": mib, "copies": reps}))
 
 
+def bench_neighborhood(cga, args):
+    """The Neighborhood baselines on ER n=4096, p=0.1, graph-seed 3: all 4096 start partitions in batches of 256,
+    without and with local search, for each local-search kernel and both G rebuilds (popcount, int8 matrix cores);
+    the reference CPU enumeration (oracle/_ref) beside it. One JSON line."""
+    graph = cga.erdos_renyi(4096, 0.1, 3)
+    rows = []
+    legs = [(False, "auto", "popcount"), (False, "auto", "mfma"), (True, "auto", "popcount"), (True, "auto", "mfma"),
+            (True, "global", "popcount"), (True, "shared", "popcount"), (True, "wide", "popcount")]
+    with cga.DeviceGraph(graph) as dg:
+        dg.solve_neighborhood(False, k=2, population=256)  # warm-up
+        dg.solve_neighborhood(False, k=2, population=256, rebuild="mfma")
+        first = {}
+        for local_search, ls, rebuild in legs:
+            t0 = time.perf_counter()
+            res = dg.solve_neighborhood(local_search, k=2, population=256, ls_kernel=ls, profile=True,
+                                        rebuild=rebuild)
+            wall = time.perf_counter() - t0
+            base = first.setdefault(local_search, res)
+            same = (res["objective"] == base["objective"] and res["accepted_moves"] == base["accepted_moves"] and
+                    bool((res["labels"] == base["labels"]).all()))
+            rows.append({"local_search": local_search, "ls_kernel": ls, "rebuild": rebuild,
+                         "descent": res["descent_name"], "seconds": round(wall, 4),
+                         "starts_per_s": round(graph.n / wall, 1), "objective": res["objective"],
+                         "accepted_moves": res["accepted_moves"], "rebuild_ms": round(res["rebuild_ms"], 3),
+                         "search_ms": round(res["search_ms"], 3), "rebuild_launches": res["rebuild_launches"],
+                         "same_as_first": bool(same)})
+            note(f"neighborhood local_search={local_search} {ls} {rebuild}: {wall:.3f} s")
+    best = min(r["seconds"] for r in rows if r["local_search"])
+    out = {"metric": "Neighborhood start partitions per second (with local search, best kernel)",
+           "value": round(graph.n / best, 1), "unit": "starts/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic", "dtype": "int16",
+           "config": {"workload": "ER n=4096, p=0.1, graph-seed 3; k=2; 4096 starts in batches of 256"},
+           "edges": graph.edges, "runs": rows}
+    ref = None if args.no_cpu else reference_solver()
+    if ref is not None:
+        import oracle
+
+        og = oracle.Graph(graph.bits, graph.n, graph.edges)
+        t0 = time.perf_counter()
+        cpu = ref.neighborhood(og, local_search=False)
+        out["cpu_baseline"] = {"value": round(graph.n / (time.perf_counter() - t0), 1), "unit": "starts/s",
+                               "cores": 1, "kind": "reference",
+                               "sample": "all 4096 starts without local search, the reference's sequential "
+                                         "enumeration (cuda/cc_verify_gpu.cu:21-35)",
+                               "bit_exact": cpu["objective"] == rows[0]["objective"]}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
     ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson request rate; 0 skips serving")
     ap.add_argument("--serve-iterations", type=int, default=20, help="iterations of the wide-path served run")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--autotune", type=int, default=0, metavar="I",
+                    help="before timing, let the autotuner time every descent plan on I iterations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (to scale byte counters) and exit")
     args = ap.parse_args()
@@ -284,12 +334,23 @@ def main():
 
     import cuda_graph_clustering_amd as cga
 
+    if args.config == "c3":
+        bench_neighborhood(cga, args)
+        return
     cfg = CONFIGS[args.config]
     steps = args.steps if args.steps else cfg["iterations"]
     graph = make_graph(cga, cfg["gen"])
     common = dict(k=cfg["k"], population=cfg["population"], gww=cfg["gww"], seed=cfg["seed"])
 
+    tuned = None
     with cga.DeviceGraph(graph) as dg:
+        if args.autotune > 0:  # before the warm-up, outside the timed region
+            cands, chosen = cga.autotune(dg, tune_iterations=args.autotune, iterations=steps, early_stop=steps,
+                                         **common)
+            tuned = {"chosen": chosen, "candidates": [{k: c[k] for k in ("descent", "threads", "vertices_per_thread",
+                                                                           "rows_in_lds", "blocks", "search_ms")}
+                                                      for c in cands]}
+            note(f"autotune: {len(cands)} plans, chose {cands[chosen]['descent']} {cands[chosen]['threads']} threads")
         if args.warmup > 0:
             dg.solve_pbils(iterations=args.warmup, early_stop=args.warmup, **common)
         t0 = time.perf_counter()
@@ -361,6 +422,7 @@ def main():
         }
     else:
         out["cpu_baseline"] = None
+    out["autotune"] = tuned
     out["batched"] = bench_batch(cga, args.batch_count, plain_leg=not args.batch_only) if args.batch_count > 0 else None
     out["serving"] = (bench_serve(cga, rate=args.serve_rate, big_iterations=args.serve_iterations)
                       if args.serve_rate > 0 else None)

@@ -17,6 +17,7 @@ CLI_PATH = os.path.join(HERE, "bin", "cc_gpu")
 
 LS_AUTO, LS_GLOBAL, LS_SHARED, LS_WIDE = 0, 1, 2, 3
 LS_KERNELS = {"auto": LS_AUTO, "global": LS_GLOBAL, "shared": LS_SHARED, "wide": LS_WIDE}
+REBUILDS = {"popcount": 0, "mfma": 1}
 DESCENTS = {0: "generic-global", 1: "generic-lds", 2: "k2-registers", 3: "k2-wide", 4: "k3-wide", 5: "registers",
             6: "mixed"}
 
@@ -44,7 +45,7 @@ class Params(ctypes.Structure):
         ("verbose", ctypes.c_int32),
         ("ls_kernel", ctypes.c_int32),
         ("profile", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("rebuild", ctypes.c_int32),
     ]
 
 
@@ -258,7 +259,7 @@ def objective_direct(graph, labels):
 
 
 def make_params(k=2, population=128, tournament=5, iterations=100, early_stop=6, perturbation=0.4, gww=0.1, seed=1,
-                time_limit_sec=0.0, verbose=False, ls_kernel="auto", profile=False):
+                time_limit_sec=0.0, verbose=False, ls_kernel="auto", profile=False, rebuild="popcount"):
     p = Params()
     lib.cc_pbils_default_params(ctypes.byref(p))
     p.k, p.population, p.tournament, p.iterations, p.early_stop = k, population, tournament, iterations, early_stop
@@ -266,6 +267,7 @@ def make_params(k=2, population=128, tournament=5, iterations=100, early_stop=6,
     p.verbose = int(bool(verbose))
     p.ls_kernel = LS_KERNELS[ls_kernel] if isinstance(ls_kernel, str) else int(ls_kernel)
     p.profile = int(bool(profile))
+    p.rebuild = REBUILDS[rebuild] if isinstance(rebuild, str) else int(rebuild)
     return p
 
 

@@ -120,6 +120,16 @@ constexpr int kRecordInitial = 0;    // offer the population after init (no iter
 constexpr int kRecordIteration = 1;  // offer `next` after a descent, then stall / stop / GWW ranking
 constexpr int kRecordBatch = 2;      // Neighborhood batch: offer `next`, add moves, strict < keeps earlier starts
 
+// ---- k = 2 rebuild on the matrix cores (cc_mfma.hip) -----------------------------------------------------------
+
+// n rounded up to the 128-byte K slabs (and 64-row tiles) of the int8 matrices.
+size_t MfmaPad(int n);
+// adj8: n_pad x n_pad bytes, adj8[v][u] = A[v][u] (zero padding).
+void LaunchExpandAdjacency(const uint64_t* bits, int n, int words, int n_pad, int8_t* adj8, hipStream_t stream);
+// G (int16), sizes untouched, intra[s] accumulated (zeroed before by LaunchMasks) for slots 0..count-1 of `run`;
+// z: MfmaPad(count) x n_pad bytes of scratch.
+void LaunchRebuildMfmaK2(const RunDev* run, int count, int n_pad, const int8_t* adj8, int8_t* z, hipStream_t stream);
+
 // ---- descent (cc_descent.hip) --------------------------------------------------------------------------------
 
 // Descent kinds (values match CC_DESCENT_* of cc_hip.h).

@@ -97,6 +97,9 @@ void CheckParams(const cc_pbils_params& p) {
   if (p.gww < 0.0 || p.gww > 0.5) Fail(CC_HIP_EINVAL, "gww must be in [0, 0.5]");
   if (p.k > kMaxClusters) Fail(CC_HIP_EINVAL, "k exceeds kMaxClusters (%d)", kMaxClusters);
   if (p.ls_kernel < CC_LS_AUTO || p.ls_kernel > CC_LS_WIDE) Fail(CC_HIP_EINVAL, "unknown ls_kernel %d", p.ls_kernel);
+  if (p.rebuild != CC_REBUILD_POPCOUNT && p.rebuild != CC_REBUILD_MFMA) {
+    Fail(CC_HIP_EINVAL, "unknown rebuild %d", p.rebuild);
+  }
 }
 
 bool WidePath(uint32_t n, const cc_pbils_params& p) {
@@ -640,6 +643,9 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   if (params.ls_kernel < CC_LS_AUTO || params.ls_kernel > CC_LS_WIDE) {
     Fail(CC_HIP_EINVAL, "unknown ls_kernel %d", params.ls_kernel);
   }
+  if (params.rebuild != CC_REBUILD_POPCOUNT && params.rebuild != CC_REBUILD_MFMA) {
+    Fail(CC_HIP_EINVAL, "unknown rebuild %d", params.rebuild);
+  }
   cc_pbils_params p = params;
   const int n = static_cast<int>(g.n);
   p.population = std::min(params.population, n);
@@ -655,16 +661,36 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   Timer* t = p.profile ? &timer : nullptr;
   LaunchResetCtl(batch.Runs(), 1, s);
   const int capacity = p.population;
+  // The matrix-core rebuild: the adjacency expanded once to bytes, a start matrix per batch.
+  const bool mfma = p.rebuild == CC_REBUILD_MFMA;
+  DeviceMemory mfma_memory;
+  int8_t* adj8 = nullptr;
+  int8_t* z = nullptr;
+  const int n_pad = static_cast<int>(MfmaPad(n));
+  if (mfma) {
+    if (spec.plan.wide) Fail(CC_HIP_EINVAL, "the matrix-core rebuild needs n <= %d", kMaxVerticesNarrow);
+    const size_t bytes = static_cast<size_t>(n_pad) * n_pad + MfmaPad(capacity) * n_pad;
+    if (bytes > FreeDeviceBytes()) Fail(CC_HIP_ENOMEM, "the matrix-core rebuild needs %s", Mebibytes(bytes).c_str());
+    adj8 = static_cast<int8_t*>(mfma_memory.AllocateBytes(static_cast<size_t>(n_pad) * n_pad));
+    z = static_cast<int8_t*>(mfma_memory.AllocateBytes(MfmaPad(capacity) * n_pad));
+    if (t) t->Begin(s);
+    LaunchExpandAdjacency(g.bits, n, static_cast<int>(g.words), n_pad, adj8, s);
+    if (t) t->End(s, &stats->rebuild_ms, &stats->rebuild_launches);
+  }
   for (int first = 0; first < n; first += capacity) {
     const int count = std::min(capacity, n - first);
     if (count != capacity) batch.SetPopulation(0, count);
     LaunchNeighborhoodLabels(batch.Runs(), n, first, count, s);
     if (t) t->Begin(s);
     LaunchMasks(batch.Runs(), batch.Prefix(batch.AllItems(), count), s);
-    for (const RebuildGroup& rg : batch.RebuildGroups()) {
-      RebuildGroup prefix = rg;
-      prefix.pairs = batch.Prefix(rg.pairs, (count + 1) / 2);
-      LaunchRebuildG(batch.Runs(), prefix, n, s);
+    if (mfma) {
+      LaunchRebuildMfmaK2(batch.Runs(), count, n_pad, adj8, z, s);
+    } else {
+      for (const RebuildGroup& rg : batch.RebuildGroups()) {
+        RebuildGroup prefix = rg;
+        prefix.pairs = batch.Prefix(rg.pairs, (count + 1) / 2);
+        LaunchRebuildG(batch.Runs(), prefix, n, s);
+      }
     }
     LaunchFinalize(batch.Runs(), batch.Prefix(batch.AllItems(), count), s);
     if (t) t->End(s, &stats->rebuild_ms, &stats->rebuild_launches);

@@ -54,8 +54,13 @@ typedef struct cc_pbils_params {
   int32_t verbose;
   int32_t ls_kernel; /* CC_LS_* */
   int32_t profile;   /* 1: time the search and rebuild kernels with HIP events (fills the *_ms fields) */
-  int32_t reserved;
+  int32_t rebuild;   /* CC_REBUILD_*: how a Neighborhood batch computes G */
 } cc_pbils_params;
+
+/* The G rebuild of a Neighborhood batch (k = 2): popcounts of adjacency words against cluster masks, or an int8
+ * matrix product on the matrix cores (n <= 32767; exact, the same G). PBILS runs always use the popcounts. */
+#define CC_REBUILD_POPCOUNT 0
+#define CC_REBUILD_MFMA 1
 
 /* Mirrors PbilsResult (reference include/cc_pbils.hpp:51-59) plus the plan and device timings. */
 typedef struct cc_pbils_stats {

@@ -1,6 +1,7 @@
 """NeighborhoodCuda / NeighborhoodWithManyLocalSearchesCuda on the GPU against the oracle's sequential enumeration,
 with the reference GPU test's coverage (cuda/cc_verify_gpu.cu:79-112): every graph with n <= 4, word- and
-block-boundary sizes, every descent kernel, a partial last batch, a batch of one and a batch larger than n."""
+block-boundary sizes, every descent kernel, a partial last batch, a batch of one and a batch larger than n. Both G
+rebuilds (popcounts and the int8 matrix product on the matrix cores) must give the same results."""
 import itertools
 
 import numpy as np
@@ -16,14 +17,14 @@ def cga():
     return m
 
 
-def compare(cga, orc, g, batch, kernel):
+def compare(cga, orc, g, batch, kernel, rebuild="popcount"):
     import oracle
 
     og = oracle.Graph(g.bits, g.n, g.edges)
     for ls in (False, True):
         want = orc.neighborhood(og, ls, threads=1)
-        got = cga.solve_neighborhood(g, ls, population=batch, ls_kernel=kernel)
-        ctx = f"n={g.n} batch={batch} kernel={kernel} ls={ls}"
+        got = cga.solve_neighborhood(g, ls, population=batch, ls_kernel=kernel, rebuild=rebuild)
+        ctx = f"n={g.n} batch={batch} kernel={kernel} ls={ls} rebuild={rebuild}"
         assert got["objective"] == want["objective"], ctx
         assert (got["labels"] == want["labels"]).all(), ctx + " labels and tie-breaking"
         assert got["objective"] == cga.objective_direct(g, got["labels"]), ctx
@@ -41,7 +42,8 @@ def test_all_graphs_up_to_four_vertices(cga, orc):
                 if (mask >> b) & 1:
                     adj[i, j] = adj[j, i] = 1
             bits = np.packbits(np.pad(adj, ((0, 0), (0, 64 - n))), axis=1, bitorder="little").view(np.uint64)
-            compare(cga, orc, cga.from_bits(bits, n), 3, "auto")
+            for rebuild in ("popcount", "mfma"):
+                compare(cga, orc, cga.from_bits(bits, n), 3, "auto", rebuild)
 
 
 @pytest.mark.parametrize("n", [31, 32, 33, 127, 129, 257])
@@ -52,10 +54,30 @@ def test_boundaries_and_kernels(cga, orc, n):
             compare(cga, orc, g, 17, kernel)
         compare(cga, orc, g, 1, "auto")
         compare(cga, orc, g, n + 3, "auto")
+        for batch in (1, 17, n + 3):
+            compare(cga, orc, g, batch, "auto", "mfma")
+
+
+def test_matrix_core_rebuild_at_full_size(cga, orc, cpu_threads):
+    """ER n=4096, p=0.1, graph-seed 3, batches of 256: the matrix-core rebuild gives the popcount results with and
+    without local search, and the enumeration without local search matches the oracle."""
+    import oracle
+
+    g = cga.erdos_renyi(4096, 0.1, 3)
+    with cga.DeviceGraph(g) as dg:
+        for ls in (False, True):
+            a = dg.solve_neighborhood(ls, k=2, population=256)
+            b = dg.solve_neighborhood(ls, k=2, population=256, rebuild="mfma")
+            for key in ("objective", "accepted_moves", "clusters_used", "local_searches"):
+                assert a[key] == b[key], (ls, key)
+            assert (a["labels"] == b["labels"]).all()
+            if not ls:
+                want = orc.neighborhood(oracle.Graph(g.bits, g.n, g.edges), False, threads=cpu_threads)
+                assert b["objective"] == want["objective"] and (b["labels"] == want["labels"]).all()
 
 
 def test_invalid_input_rejected(cga):
-    for kw, n in (({}, 0), ({"k": 3}, 1), ({"population": 0}, 1)):
+    for kw, n in (({}, 0), ({"k": 3}, 1), ({"population": 0}, 1), ({"rebuild": 7}, 5)):
         g = cga.erdos_renyi(n, 0.5, 1)
         for ls in (False, True):
             with pytest.raises(cga.CcError) as e:
