@@ -67,7 +67,9 @@ def descent_bytes_per_move(n, k, descent, mean_degree):
     """Algorithmic bytes one accepted move streams: the slot's state scanned once plus the adjacency row of the
     moved vertex (and, for the packed wide kernels, the write-back of the neighbours' words). The state is G and
     the labels (k n |G| + n) for the generic and register kernels, e and the label bit (4 n) for the k = 2 register
-    kernel, and one packed word per vertex (4 n for k = 2, 8 n for k = 3) for the wide kernels."""
+    kernel, and one packed word per vertex (4 n for k = 2, 8 n for k = 3) for the wide kernels. The chunk-summary
+    kernel reads the row and re-reads only the 64-vertex chunks holding the moved vertex and its neighbours (at most
+    mean degree + 1 chunks of 512 bytes, written back)."""
     words = max(1, (n + 63) // 64)
     row = 8 * words
     if descent == "k2-registers":
@@ -76,6 +78,8 @@ def descent_bytes_per_move(n, k, descent, mean_degree):
         return 4 * n + row + 4 * mean_degree
     if descent == "k3-wide":
         return 8 * n + row + 8 * mean_degree
+    if descent == "k3-chunk":
+        return row + 2 * 512 * min(mean_degree + 1, words)
     gbytes = 2 if n <= 32767 else 4
     return k * n * gbytes + n + row
 

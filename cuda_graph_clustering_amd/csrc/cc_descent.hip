@@ -770,6 +770,185 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   }
 }
 
+// Wide path, k = 3, with chunk summaries. The vertices are cut into chunks of 64 (one adjacency word, one wave).
+// For every chunk and every pair (label a, target c != a) a table in LDS holds the chunk's best candidate of the
+// pair,
+//   key = ((G_c - G_a + 2^20) << 6) | (63 - u % 64),   the largest G_c - G_a, ties to the smaller vertex u,
+// or 0 if no vertex of the chunk has label a. Since delta(u, a -> c) = (s_c - s_a + 1) - 2 (G_c - G_a) and the
+// sizes are the same for the whole pair, the pair's best move is its largest key over the chunks (the first chunk
+// on ties), and the move of the descent (smallest delta, then vertex, then target; src/cc_pbils_cpu.cpp:99-118) is
+// the smallest PackMove over the six pairs' best keys, the sizes applied there. A move of v changes G only for v's
+// neighbours and the label only of v, so only the chunks of the non-zero words of v's row and v's own chunk are
+// recomputed: a few chunks per move on a sparse graph instead of all n vertices. The packed vertex words are those
+// of KernelDescentK3Wide. Pairs: p = 2 a + i, c = the i-th cluster other than a.
+constexpr int kPairs3 = 6;
+
+__device__ __forceinline__ constexpr int PairTarget3(int p) {
+  return (p >> 1) == 0 ? 1 + (p & 1) : ((p >> 1) == 1 ? 2 * (p & 1) : (p & 1));
+}
+
+__device__ __forceinline__ uint32_t WaveMax32(uint32_t x) { return ~WaveMin(~x); }
+
+template <int T>
+__device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
+  Ctl* ctl = r.ctl;
+  constexpr int kWaves = T / kWave;
+  const int n = r.n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = WaveIndex();
+  const int chunks = (n + 63) >> 6;
+  __shared__ uint64_t partial[2][kWaves];
+  __shared__ int32_t dirty_count[2];
+  __shared__ int32_t src_sh;
+  extern __shared__ __align__(16) uint8_t dyn[];
+  uint32_t* table = reinterpret_cast<uint32_t*>(dyn);               // [kPairs3][chunks]
+  int32_t* dirty = reinterpret_cast<int32_t*>(table + kPairs3 * chunks);  // [2][chunks]
+  if (tid == 0) {
+    src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
+    dirty_count[0] = 0;
+    dirty_count[1] = 0;
+  }
+  __syncthreads();
+  const int src = src_sh;
+  const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
+  const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
+  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
+  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * n);
+  int sz[3] = {sizes_in[0], sizes_in[1], sizes_in[2]};
+  const int64_t f_start = Uniform(r.pop.f[src]);
+
+  // The six pair keys of a chunk from its lanes' words, into the table.
+  auto summarize = [&](int chunk, uint64_t w, bool valid) {
+    const int a = valid ? static_cast<int>(w >> kK3LabelShift) : 3;
+    const int g[3] = {static_cast<int>(w & kK3Mask), static_cast<int>((w >> kK3Bits) & kK3Mask),
+                      static_cast<int>((w >> (2 * kK3Bits)) & kK3Mask)};
+    const int ga = a == 0 ? g[0] : (a == 1 ? g[1] : g[2]);
+    const uint32_t tie = static_cast<uint32_t>(63 - lane);
+#pragma unroll
+    for (int p = 0; p < kPairs3; ++p) {
+      const uint32_t key =
+          a == (p >> 1) ? (static_cast<uint32_t>(g[PairTarget3(p)] - ga + (1 << kK3Bits)) << 6) | tie : 0u;
+      const uint32_t best = WaveMax32(key);
+      if (lane == 0) table[p * chunks + chunk] = best;
+    }
+  };
+
+  for (int chunk = wave; chunk < chunks; chunk += kWaves) {
+    const int u = chunk * kWave + lane;
+    uint64_t w = 0;
+    if (u < n) {
+      w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
+          static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) | static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
+      st[u] = w;
+    }
+    summarize(chunk, w, u < n);
+  }
+
+  // The thread's candidate: over its chunks of the table, each pair's best entry, as moves. Across chunks only the
+  // gain (key >> 6) is compared and the first chunk wins a tie: the low bits order vertices within one chunk only,
+  // and every vertex of an earlier chunk is smaller than every vertex of a later one.
+  auto candidate = [&]() {
+    uint64_t best = kNoMove;
+#pragma unroll
+    for (int p = 0; p < kPairs3; ++p) {
+      uint32_t m = 0;
+      int mc = 0;
+      for (int c = tid; c < chunks; c += T) {
+        const uint32_t k = table[p * chunks + c];
+        if ((k >> 6) > (m >> 6)) {
+          m = k;
+          mc = c;
+        }
+      }
+      if (m) {
+        const int a = p >> 1, to = PairTarget3(p);
+        const int x = static_cast<int>(m >> 6) - (1 << kK3Bits);
+        const int d = (sz[to] - sz[a] + 1) - 2 * x;
+        if (d < 0) {
+          const int u = mc * kWave + 63 - static_cast<int>(m & 63u);
+          best = Min(best, PackMove(d, u, a, to));
+        }
+      }
+    }
+    return best;
+  };
+
+  __syncthreads();  // the table is complete
+  uint64_t key = candidate();
+  int64_t objective = f_start;
+  int64_t applied = 0;
+  const int64_t cap = MoveCap(f_start, n);
+  int parity = 0;
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  while (true) {
+    const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
+    if (best == kNoMove) break;
+    const int delta = static_cast<int>(static_cast<uint32_t>(best >> 32)) - kDeltaOffset;
+    const uint32_t mv = static_cast<uint32_t>(best);
+    const int v = static_cast<int>(mv >> 12);
+    const int from = static_cast<int>((mv >> 6) & 63u);
+    const int to = static_cast<int>(mv & 63u);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sz[c] += (c == to) - (c == from);
+    const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
+    const uint64_t* row = row_base + static_cast<size_t>(v) * row_words;
+    // the chunks to recompute: every non-zero word of v's row, and v's own chunk
+    int32_t* list = dirty + parity * chunks;
+    for (int c = tid; c < chunks; c += T) {
+      if (row[c] != 0ull || c == (v >> 6)) list[atomicAdd(&dirty_count[parity], 1)] = c;
+    }
+    __syncthreads();
+    const int count = dirty_count[parity];
+    for (int i = wave; i < count; i += kWaves) {
+      const int chunk = list[i];
+      const uint64_t word = row[chunk];  // wave-uniform
+      const int u = chunk * kWave + lane;
+      uint64_t w = 0;
+      if (u < n) {
+        w = st[u];
+        const bool neighbour = (word >> lane) & 1ull;
+        if (neighbour) w += inc;
+        if (u == v) w = (w & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
+        if (neighbour || u == v) st[u] = w;
+      }
+      summarize(chunk, w, u < n);
+    }
+    // the other list's counter was last read before this move's first barrier
+    if (tid == 0) dirty_count[parity ^ 1] = 0;
+    parity ^= 1;
+    __syncthreads();  // the table is current
+    key = candidate();
+    objective += delta;
+    if (++applied >= cap) {
+      if (tid == 0) {
+        ctl->error = 1;
+        ctl->stopped = 1;
+      }
+      break;
+    }
+  }
+  uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
+  int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
+  for (int u = tid; u < n; u += T) {
+    const uint64_t w = st[u];
+    g_out[u] = static_cast<int32_t>(w & kK3Mask);
+    g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
+    g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
+    lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
+  }
+  if (tid == 0) {
+    int32_t* sizes = r.next.sizes + static_cast<size_t>(s) * 3;
+    sizes[0] = sz[0];
+    sizes[1] = sz[1];
+    sizes[2] = sz[2];
+    r.next.f[s] = objective;
+    r.moves[s] = applied;
+  }
+}
+
 constexpr size_t kLdsBytes = 160 * 1024;
 // Static LDS of the descent kernels (partials, sizes, source slot, claim), rounded up.
 constexpr size_t kStaticLdsReserve = 2048;
@@ -816,6 +995,11 @@ __global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, Ite
 template <int T>
 __global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list) {
   Persistent(runs, list, [](const RunDev& r, int s) { DescentK3WideBody<T>(r, s); });
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) KernelDescentK3Chunk(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentK3ChunkBody<T>(r, s); });
 }
 
 // The mixed descent: one persistent 1024-thread launch for a batch whose runs use different register kernels. The
@@ -1012,6 +1196,11 @@ bool Dispatch(const Go& go) {
       return p.wide && p.k == 2 && GoWide<WideK2>(go);
     case Descent::kK3Wide:
       return p.wide && p.k == 3 && GoWide<WideK3>(go);
+    case Descent::kK3Chunk:
+      if (!p.wide || p.k != 3 || p.lds_bytes == 0) return false;
+      if (p.threads == 256) return go.Run(KernelDescentK3Chunk<256>, 256, p.lds_bytes);
+      if (p.threads == 1024) return go.Run(KernelDescentK3Chunk<1024>, 1024, p.lds_bytes);
+      return false;
     case Descent::kMixed:
       return !p.wide && p.threads == kMixedThreads && go.Run(KernelDescentMixed, kMixedThreads, 0);
   }
@@ -1034,6 +1223,11 @@ void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan&
 
 size_t GenericLdsBytes(int n, int k, int wide) {
   return Align16((wide ? 4 : 2) * static_cast<size_t>(k) * n) + Align16(static_cast<size_t>(n));
+}
+
+size_t K3ChunkLdsBytes(int n) {
+  const size_t chunks = (static_cast<size_t>(n) + 63) / 64;
+  return (kPairs3 + 2) * chunks * sizeof(uint32_t);  // pair table + two dirty lists
 }
 
 int MixedVariant(int n, int k, DescentPlan* plan) {

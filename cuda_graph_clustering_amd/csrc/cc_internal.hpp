@@ -141,6 +141,7 @@ enum class Descent : int {
   kK3Wide = 4,
   kRegisters = 5,  // k in [3, 8], narrow, state in registers
   kMixed = 6,      // a batch launch over runs of different register kernels (1024 threads), by RunDev::variant
+  kK3Chunk = 7,    // k = 3 wide, packed words plus per-chunk pair summaries in LDS (K3ChunkLdsBytes)
 };
 
 struct DescentPlan {
@@ -165,6 +166,7 @@ void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan&
 bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
+size_t K3ChunkLdsBytes(int n);
 int RegistersVpt(int n, int k, int* threads);
 // The mixed-descent variant that runs a register plan of (k, n) with 1024 threads, or -1 if none; `plan` receives
 // the equivalent 1024-thread plan.

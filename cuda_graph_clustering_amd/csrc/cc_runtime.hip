@@ -132,10 +132,14 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
     Fail(CC_HIP_EINVAL, "state needs %zu bytes of LDS, the device allows %zu", lds, DescentLdsLimit());
   }
   // Wide path: the packed k = 2 / k = 3 kernels unless the generic kernel is asked for
-  // (reference cuda/cc_pbils_gpu.cu:84-96).
+  // (reference cuda/cc_pbils_gpu.cu:84-96); for k = 3 the chunk-summary variant while its table fits in LDS.
   if (wide && p.ls_kernel != CC_LS_GLOBAL && (k == 2 || k == 3)) {
     plan.descent = k == 2 ? Descent::kK2Wide : Descent::kK3Wide;
     plan.threads = n >= 8192 ? 1024 : 256;
+    if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit()) {
+      plan.descent = Descent::kK3Chunk;
+      plan.lds_bytes = K3ChunkLdsBytes(n);
+    }
     return plan;
   }
   if (p.ls_kernel == CC_LS_AUTO && !wide && k >= 3) {
@@ -278,7 +282,8 @@ void LayoutRun(const RunSpec& s, RunDev& r, Arena& a) {
   a.Take(r.order, P);
   a.Take(r.masks, P * k * s.graph->words);
   a.Take(r.intra, P);
-  a.Take(r.work, s.plan.descent == Descent::kK3Wide ? P * n : 0);
+  const bool packed3 = s.plan.descent == Descent::kK3Wide || s.plan.descent == Descent::kK3Chunk;
+  a.Take(r.work, packed3 ? P * n : 0);
 }
 
 }  // namespace
@@ -541,6 +546,7 @@ const char* DescentName(Descent d) {
     case Descent::kK3Wide: return "k = 3, packed words";
     case Descent::kRegisters: return "k = 3..8, registers";
     case Descent::kMixed: return "mixed register kernels";
+    case Descent::kK3Chunk: return "k = 3, packed words with chunk summaries";
   }
   return "?";
 }

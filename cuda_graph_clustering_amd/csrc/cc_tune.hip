@@ -97,6 +97,13 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
         p.blocks = b;
         AddIfSupported(out, p);
       }
+      if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit()) {
+        DescentPlan p = base;
+        p.descent = Descent::kK3Chunk;
+        p.threads = t;
+        p.lds_bytes = K3ChunkLdsBytes(n);
+        AddIfSupported(out, p);
+      }
     }
   }
   const size_t lds = GenericLdsBytes(n, k, wide ? 1 : 0);
@@ -221,8 +228,8 @@ int LoadTuned(const char* path) {
     p.wide = wide;
     p.blocks = blocks;
     // any single-run kernel (the mixed batch kernel is chosen per batch, not per shape)
-    const bool valid = descent >= 0 && descent <= static_cast<int>(Descent::kRegisters) && lds <= DescentLdsLimit() &&
-                       DescentSupported(p);
+    const bool valid = descent >= 0 && descent <= static_cast<int>(Descent::kK3Chunk) &&
+                       p.descent != Descent::kMixed && lds <= DescentLdsLimit() && DescentSupported(p);
     if (!valid) {
       std::fclose(f);
       Fail(CC_HIP_EINVAL, "%s: entry %d names no supported descent plan", path, line);

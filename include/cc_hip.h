@@ -91,6 +91,7 @@ typedef struct cc_pbils_stats {
 #define CC_DESCENT_K3_WIDE 4
 #define CC_DESCENT_REGISTERS 5 /* k in [3, 8], n <= 32767: state in VGPRs */
 #define CC_DESCENT_MIXED 6     /* batched: one launch over runs of different register kernels */
+#define CC_DESCENT_K3_CHUNK 7  /* k = 3 wide: packed words plus per-chunk move summaries in LDS */
 
 void cc_pbils_default_params(cc_pbils_params* params);
 

@@ -105,11 +105,12 @@ def wide_graph(cga):
 
 @pytest.mark.parametrize("k,ls_kernel", [(2, "auto"), (3, "auto"), (3, "global"), (4, "auto")])
 def test_wide_path_full_size_trajectory(cga, orc, cpu_threads, wide_graph, k, ls_kernel):
-    """n above the 16-bit range: int32 G, the packed k = 2 / k = 3 kernels (auto) or the generic global kernel."""
+    """n above the 16-bit range: int32 G, the packed k = 2 kernel and the k = 3 chunk-summary kernel (auto) or the
+    generic global kernel. (The streaming k = 3 kernel runs beside the others in the autotuner test.)"""
     got, want = run_both(cga, orc, wide_graph, cpu_threads, k=k, population=4, iterations=2, early_stop=2,
                          gww=0.25, seed=3, ls_kernel=ls_kernel)
     check_same(got, want, f"wide k={k} {ls_kernel}")
-    expected = {(2, "auto"): "k2-wide", (3, "auto"): "k3-wide", (3, "global"): "generic-global",
+    expected = {(2, "auto"): "k2-wide", (3, "auto"): "k3-chunk", (3, "global"): "generic-global",
                 (4, "auto"): "generic-global"}[(k, ls_kernel)]
     assert got["descent_name"] == expected
 
@@ -228,3 +229,23 @@ def test_device_graph_repeated_solves_are_identical(cga):
     for key in KEYS:
         assert a[key] == b[key] == c[key]
     assert (a["labels"] == b["labels"]).all() and (a["labels"] == c["labels"]).all()
+
+
+@pytest.mark.parametrize("threads", [256, 1024])
+def test_wide_chunk_kernel_with_several_chunks_per_thread(cga, orc, cpu_threads, tmp_path, threads):
+    """The k = 3 chunk-summary kernel when a thread owns several 64-vertex chunks (n = 70000: 1094 chunks): across
+    chunks only the gain may decide, the earlier chunk winning a tie, or vertex order breaks. Both workgroup sizes
+    are forced through a tuned-plan entry and must follow the oracle."""
+    g = cga.chung_lu(70000, 2.5, 8.0, 3)
+    kw = dict(k=3, population=4, iterations=2, early_stop=2, seed=3, gww=0.1)
+    lds = (70000 + 63) // 64 * 32
+    table = tmp_path / "plan.txt"
+    table.write_text(f"70000 3 4 1 7 {threads} 0 0 {lds} 0\n")
+    cga.tune_clear()
+    try:
+        assert cga.tune_load(table) == 1
+        got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    finally:
+        cga.tune_clear()
+    check_same(got, want, f"chunk kernel {threads} threads")
+    assert (got["descent_name"], got["threads"]) == ("k3-chunk", threads)
