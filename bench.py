@@ -29,8 +29,8 @@ sys.path.insert(0, ROOT)
 # accepted move). A timed solve of the same length has the same trajectory, so its launches move the same bytes;
 # other lengths are scaled by their accepted moves. Commands and counter values: profiles/descent_*.txt.
 MEASURED_TRAFFIC = {
-    "c2": (200, 808289686, 808289686 / (101473818 / 200)),
-    "c5": (2, 5251166771568, 813040.0),
+    "c2": (200, 433203205, 853.8),
+    "c5": (2, 195321610816, 30241.7),
 }
 
 METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"

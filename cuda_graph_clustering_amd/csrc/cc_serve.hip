@@ -52,7 +52,11 @@ struct Active {
   Clock::time_point admitted;
   std::unique_ptr<DeviceGraph> graph;
   std::unique_ptr<RunState> state;
+  bool done = false;  // result delivered; the run stays in the launch set (skipped by every kernel) until a rebuild
 };
+
+// Iterations a lane advances between host synchronisations while no request waits for admission.
+constexpr int kIdleChunk = 4;
 
 size_t GraphBytes(uint32_t n) {
   return static_cast<size_t>(n) * Words64(n) * 8 + kRowTailBytes + static_cast<size_t>(n) * 4;
@@ -191,6 +195,7 @@ class Server {
     std::vector<Active> active;
     std::unique_ptr<LaunchSet> set;
     bool changed = false;
+    size_t finished = 0;  // runs of `active` whose result is delivered
     while (true) {
       // admit
       std::vector<Request> admitted = Admit(lane, static_cast<int>(active.size()));
@@ -224,6 +229,30 @@ class Server {
         start.Start();
         changed = true;
       }
+      // Finished runs stay in the launch set (every kernel skips a stopped run) until new runs rebuild it anyway,
+      // they make up half of it, or queued requests wait for their memory; then it goes back to the budget.
+      bool waiting = false;
+      {
+        std::lock_guard<std::mutex> lock(mutex_);
+        waiting = !lanes_[lane].pending.empty();
+      }
+      if (changed || 2 * finished > active.size() || (finished > 0 && waiting)) {
+        std::vector<Active> keep;
+        for (Active& a : active) {
+          if (!a.done) {
+            keep.push_back(std::move(a));
+            continue;
+          }
+          const size_t bytes = a.request.footprint;
+          a.state.reset();
+          a.graph.reset();
+          Release(bytes);
+        }
+        active = std::move(keep);
+        finished = 0;
+        set.reset();
+        changed = !active.empty();
+      }
       {
         std::lock_guard<std::mutex> lock(mutex_);
         lanes_[lane].active = static_cast<int64_t>(active.size());
@@ -243,17 +272,18 @@ class Server {
         set = std::make_unique<LaunchSet>(states, s);
         changed = false;
       }
-      // one round: one iteration of every active run
+      // one round: one iteration of every active run, or a few while nobody waits for admission
+      int chunk = kIdleChunk;
+      {
+        std::lock_guard<std::mutex> lock(mutex_);
+        if (!lanes_[lane].pending.empty()) chunk = 1;
+      }
       try {
-        set->EnqueueIteration(nullptr, nullptr);
+        for (int i = 0; i < chunk; ++i) set->EnqueueIteration(nullptr, nullptr);
         CC_HIP_CHECK(hipGetLastError());
         const Ctl* ctl = set->PullCtl();
-        std::vector<Active> keep;
         for (size_t i = 0; i < active.size(); ++i) {
-          if (!ctl[i].stopped) {
-            keep.push_back(std::move(active[i]));
-            continue;
-          }
+          if (active[i].done || !ctl[i].stopped) continue;
           Result r;
           Active& a = active[i];
           const uint32_t n = a.request.n;
@@ -279,26 +309,22 @@ class Server {
           r.latency = Since(a.request.arrival);
           r.queued = std::chrono::duration<double>(a.admitted - a.request.arrival).count();
           r.stats.seconds = std::chrono::duration<double>(Clock::now() - a.admitted).count();
-          const uint64_t id = a.request.id;
-          const size_t bytes = a.request.footprint;
-          a.state.reset();
-          a.graph.reset();
-          Release(bytes);
-          Complete(id, std::move(r));
-          changed = true;
-        }
-        // keep owns every run still going (moved above), finished or not this round
-        active = std::move(keep);
-        if (changed) set.reset();
-      } catch (const std::exception& e) {
-        for (Active& a : active) {
-          Result r;
-          r.code = CC_HIP_EDEVICE;
-          r.error = e.what();
-          Release(a.request.footprint);
+          a.done = true;
+          ++finished;
           Complete(a.request.id, std::move(r));
         }
+      } catch (const std::exception& e) {
+        for (Active& a : active) {
+          if (!a.done) {
+            Result r;
+            r.code = CC_HIP_EDEVICE;
+            r.error = e.what();
+            Complete(a.request.id, std::move(r));
+          }
+          Release(a.request.footprint);
+        }
         active.clear();
+        finished = 0;
         set.reset();
       }
     }
