@@ -322,6 +322,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson request rate; 0 skips serving")
     ap.add_argument("--serve-iterations", type=int, default=20, help="iterations of the wide-path served run")
+    ap.add_argument("--serve-low-rate", type=float, default=15.0,
+                    help="second serving leg at this rate, below capacity; 0 skips it")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--autotune", type=int, default=0, metavar="I",
                     help="before timing, let the autotuner time every descent plan on I iterations")
@@ -430,6 +432,9 @@ def main():
     out["batched"] = bench_batch(cga, args.batch_count, plain_leg=not args.batch_only) if args.batch_count > 0 else None
     out["serving"] = (bench_serve(cga, rate=args.serve_rate, big_iterations=args.serve_iterations)
                       if args.serve_rate > 0 else None)
+    # The same scenario at a rate below the device's batched capacity, where latency is service time, not queueing.
+    out["serving_below_capacity"] = (bench_serve(cga, rate=args.serve_low_rate, big_iterations=args.serve_iterations)
+                                     if args.serve_rate > 0 and args.serve_low_rate > 0 else None)
     print(json.dumps(out))
 
 
