@@ -101,7 +101,7 @@ void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream);
 void LaunchDegrees(const RunDev* run, int n, hipStream_t stream);
 void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream);
 // G, sizes and f of `pop` for the items; `pairs` lists (run, first slot) of slot pairs of the same run, grouped by
-// the cluster chunk (kc = 2, 4 or 8) and G width of their run.
+// the clusters per block (kc = k for k in [2, 8], 8 above) and G width of their run.
 struct RebuildGroup {
   ItemList pairs;
   int kc;

@@ -365,13 +365,17 @@ template <typename GT>
 static void LaunchRebuildGT(const RunDev* runs, const RebuildGroup& g, int max_n, hipStream_t stream) {
   ForChunksY(g.pairs.items, g.pairs.count, [&](const Item* chunk, int count) {
     const dim3 grid(Blocks(max_n, kRebuildRows), count, 1);
-    if (g.kc == 2) {
-      KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, chunk);
-    } else if (g.kc == 4) {
-      KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, chunk);
-    } else {
-      const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
-      KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, chunk);
+    switch (g.kc) {  // k <= 8: exactly k clusters per block; k > 8: chunks of 8 along grid z
+      case 2: KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      case 3: KernelRebuildG<GT, 3><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      case 4: KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      case 5: KernelRebuildG<GT, 5><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      case 6: KernelRebuildG<GT, 6><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      case 7: KernelRebuildG<GT, 7><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
+      default: {
+        const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
+        KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, chunk);
+      }
     }
   });
 }

@@ -242,7 +242,8 @@ class Arena {
   size_t used_ = 0;
 };
 
-int KcOf(int k) { return k <= 2 ? 2 : (k <= 4 ? 4 : 8); }
+// Clusters per rebuild block: exactly k up to 8 (no idle popcounts), chunks of 8 above.
+int KcOf(int k) { return k <= 2 ? 2 : (k <= 8 ? k : 8); }
 
 // Expected relative cost of one descent of a run, to order the persistent queues largest first.
 double DescentCost(const RunSpec& s) {
