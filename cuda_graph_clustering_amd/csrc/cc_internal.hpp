@@ -98,6 +98,7 @@ __host__ __device__ inline int64_t MoveCap(int64_t f_start, int n) {
 
 void LaunchResetCtl(const RunDev* runs, int nruns, hipStream_t stream);
 void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream);
+void LaunchGatherCtl(const RunDev* runs, int nruns, Ctl* out, hipStream_t stream);
 void LaunchDegrees(const RunDev* run, int n, hipStream_t stream);
 void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream);
 // G, sizes and f of `pop` for the items; `pairs` lists (run, first slot) of slot pairs of the same run, grouped by

@@ -27,6 +27,12 @@ __global__ void KernelResetCtl(const RunDev* runs, int nruns) {
   *runs[i].ctl = c;
 }
 
+// out[i] = *runs[i].ctl, so that the host reads every run's control block with one transfer.
+__global__ void KernelGatherCtl(const RunDev* runs, int nruns, Ctl* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nruns) out[i] = *runs[i].ctl;
+}
+
 __global__ void KernelResetQueues(int32_t* queues, int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) queues[i] = 0;
@@ -340,6 +346,10 @@ void ForChunksY(const Item* items, int count, Launch&& launch) {
 
 void LaunchResetCtl(const RunDev* runs, int nruns, hipStream_t stream) {
   KernelResetCtl<<<Blocks(nruns, kBlock), kBlock, 0, stream>>>(runs, nruns);
+}
+
+void LaunchGatherCtl(const RunDev* runs, int nruns, Ctl* out, hipStream_t stream) {
+  KernelGatherCtl<<<Blocks(nruns, kBlock), kBlock, 0, stream>>>(runs, nruns, out);
 }
 
 void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream) {

@@ -38,16 +38,114 @@ void Fail(int code, const char* fmt, ...) {
 
 std::string Mebibytes(size_t bytes) { return std::to_string((bytes + (1 << 20) - 1) >> 20) + " MiB"; }
 
+DevicePool::~DevicePool() {
+  for (auto& [p, bytes] : slabs_) (void)hipFree(p);
+}
+
+void* DevicePool::Take(size_t bytes) {
+  const size_t need = (bytes + 255) & ~static_cast<size_t>(255);
+  std::lock_guard<std::mutex> lock(mutex_);
+  auto it = free_.begin();
+  while (it != free_.end() && it->second < need) ++it;
+  if (it == free_.end()) {
+    // a new slab: the default size, or a whole one for a larger block (rounded to 2 MiB pages)
+    const size_t slab = std::max(slab_bytes_, (need + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1));
+    void* p = nullptr;
+    const hipError_t status = hipMalloc(&p, slab);
+    if (status != hipSuccess) {
+      (void)hipGetLastError();
+      Fail(CC_HIP_ENOMEM, "device pool: a new slab of %s failed (%s reserved, %s in use): %s", Mebibytes(slab).c_str(),
+           Mebibytes(reserved_).c_str(), Mebibytes(in_use_).c_str(), hipGetErrorString(status));
+    }
+    slabs_.push_back({static_cast<uint8_t*>(p), slab});
+    reserved_ += slab;
+    it = free_.insert({static_cast<uint8_t*>(p), slab}).first;
+  }
+  uint8_t* block = it->first;
+  const size_t left = it->second - need;
+  free_.erase(it);
+  if (left > 0) free_.insert({block + need, left});
+  used_.insert({block, need});
+  in_use_ += need;
+  return block;
+}
+
+void DevicePool::Give(void* p) {
+  std::lock_guard<std::mutex> lock(mutex_);
+  auto u = used_.find(static_cast<uint8_t*>(p));
+  if (u == used_.end()) return;
+  uint8_t* block = u->first;
+  size_t bytes = u->second;
+  used_.erase(u);
+  in_use_ -= bytes;
+  auto is_slab_start = [&](const uint8_t* a) {
+    for (const auto& s : slabs_) {
+      if (s.first == a) return true;
+    }
+    return false;
+  };
+  // merge with the free block after it and the one before it, never across the start of a slab
+  auto next = free_.find(block + bytes);
+  if (next != free_.end() && !is_slab_start(next->first)) {
+    bytes += next->second;
+    free_.erase(next);
+  }
+  auto after = free_.lower_bound(block);
+  if (after != free_.begin() && !is_slab_start(block)) {
+    auto prev = std::prev(after);
+    if (prev->first + prev->second == block) {
+      prev->second += bytes;
+      return;
+    }
+  }
+  free_.insert({block, bytes});
+}
+
+size_t DevicePool::reserved() const {
+  std::lock_guard<std::mutex> lock(mutex_);
+  return reserved_;
+}
+
+size_t DevicePool::in_use() const {
+  std::lock_guard<std::mutex> lock(mutex_);
+  return in_use_;
+}
+
 DeviceMemory::~DeviceMemory() {
-  for (void* p : blocks_) (void)hipFree(p);
+  for (void* p : blocks_) {
+    if (pool_) {
+      pool_->Give(p);
+    } else {
+      (void)hipFree(p);
+    }
+  }
 }
 
 void* DeviceMemory::AllocateBytes(size_t bytes) {
   if (bytes == 0) return nullptr;
   void* p = nullptr;
-  CC_HIP_CHECK(hipMalloc(&p, bytes));
+  if (pool_) {
+    p = pool_->Take(bytes);
+  } else {
+    CC_HIP_CHECK(hipMalloc(&p, bytes));
+  }
   blocks_.push_back(p);
   return p;
+}
+
+PinnedBuffer::~PinnedBuffer() {
+  for (void* p : buffers_) (void)hipHostFree(p);
+}
+
+void* PinnedBuffer::Reserve(size_t bytes) {
+  if (bytes > capacity_ || buffers_.empty()) {
+    const size_t grown = std::max(bytes, 2 * capacity_);
+    void* p = nullptr;
+    CC_HIP_CHECK(hipHostMalloc(&p, std::max<size_t>(grown, 64), hipHostMallocDefault));
+    buffers_.push_back(p);
+    capacity_ = std::max<size_t>(grown, 64);
+  }
+  return buffers_.back();
 }
 
 uint32_t Words64(uint32_t n) { return std::max<uint32_t>(1, (n + 63) / 64); }
@@ -58,7 +156,7 @@ size_t FreeDeviceBytes() {
   return free_bytes;
 }
 
-void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream) {
+void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream, DevicePool* pool) {
   if (n == 0) Fail(CC_HIP_EINVAL, "graph has no vertices");
   if (n > static_cast<uint32_t>(kMaxVerticesWide)) {
     Fail(CC_HIP_EINVAL, "n exceeds %d, the range of the packed move", kMaxVerticesWide);
@@ -68,11 +166,15 @@ void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hi
   g.words = Words64(n);
   g.edges = edges;
   const size_t count = static_cast<size_t>(n) * g.words;
-  const size_t bytes = count * 8 + kRowTailBytes + static_cast<size_t>(n) * 4 + sizeof(RunDev);
-  const size_t free_bytes = FreeDeviceBytes();
-  if (bytes > free_bytes) {
-    Fail(CC_HIP_ENOMEM, "graph needs %s of device memory, %s free", Mebibytes(bytes).c_str(),
-         Mebibytes(free_bytes).c_str());
+  if (pool) {
+    g.memory.UsePool(pool);
+  } else {
+    const size_t bytes = count * 8 + kRowTailBytes + static_cast<size_t>(n) * 4 + sizeof(RunDev);
+    const size_t free_bytes = FreeDeviceBytes();
+    if (bytes > free_bytes) {
+      Fail(CC_HIP_ENOMEM, "graph needs %s of device memory, %s free", Mebibytes(bytes).c_str(),
+           Mebibytes(free_bytes).c_str());
+    }
   }
   g.bits = static_cast<uint64_t*>(g.memory.AllocateBytes(count * sizeof(uint64_t) + kRowTailBytes));
   g.deg = static_cast<int32_t*>(g.memory.AllocateBytes(static_cast<size_t>(n) * sizeof(int32_t)));
@@ -296,12 +398,16 @@ size_t RunState::Footprint(const RunSpec& spec) {
   return a.used();
 }
 
-RunState::RunState(const RunSpec& spec, hipStream_t stream) : spec_(spec) {
+RunState::RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool) : spec_(spec) {
   bytes_ = Footprint(spec_);
-  const size_t free_bytes = FreeDeviceBytes();
-  if (bytes_ > free_bytes) {
-    Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes_).c_str(),
-         Mebibytes(free_bytes).c_str());
+  if (pool) {
+    memory_.UsePool(pool);
+  } else {
+    const size_t free_bytes = FreeDeviceBytes();
+    if (bytes_ > free_bytes) {
+      Fail(CC_HIP_ENOMEM, "run needs %s of device memory, %s free", Mebibytes(bytes_).c_str(),
+           Mebibytes(free_bytes).c_str());
+    }
   }
   uint8_t* base = static_cast<uint8_t*>(memory_.AllocateBytes(bytes_));
   // Start from zeros: e.g. a Neighborhood batch without local search sums a moves[] that no descent writes.
@@ -332,9 +438,11 @@ RunState::RunState(const RunSpec& spec, hipStream_t stream) : spec_(spec) {
   r.variant = spec_.variant;
 }
 
-LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream) : states_(states), stream_(stream) {
+LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, DevicePool* pool, PinnedBuffer* pinned)
+    : states_(states), stream_(stream) {
   const int nruns = static_cast<int>(states_.size());
   if (nruns == 0) Fail(CC_HIP_EINVAL, "empty batch");
+  if (pool) memory_.UsePool(pool);
   host_runs_.resize(nruns);
   for (int i = 0; i < nruns; ++i) {
     host_runs_[i] = states_[i]->dev();
@@ -372,6 +480,7 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream) :
     for (auto& kv : pairs) a.Take(pair_tables[i++], kv.second.size());
     for (size_t g = 0; g < groups.size(); ++g) a.Take(group_tables[g], groups[g].second.size());
     a.Take(queues_, std::max<size_t>(groups.size(), 1));
+    a.Take(ctl_table_, nruns);
   };
   Arena sizing;
   layout(sizing);
@@ -407,12 +516,17 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream) :
                                 hipMemcpyHostToDevice, stream_));
     descent_.push_back(d);
   }
-  CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), sizeof(Ctl) * nruns, hipHostMallocDefault));
+  if (pinned) {
+    host_ctl_ = static_cast<Ctl*>(pinned->Reserve(sizeof(Ctl) * nruns));
+  } else {
+    CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), sizeof(Ctl) * nruns, hipHostMallocDefault));
+    own_host_ctl_ = true;
+  }
   CC_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 LaunchSet::~LaunchSet() {
-  if (host_ctl_) (void)hipHostFree(host_ctl_);
+  if (host_ctl_ && own_host_ctl_) (void)hipHostFree(host_ctl_);
 }
 
 void LaunchSet::Rebuild(Timer* timer, cc_pbils_stats* stats) {
@@ -453,8 +567,13 @@ void LaunchSet::SetLdsAttributes() {
 }
 
 const Ctl* LaunchSet::PullCtl() {
-  for (size_t i = 0; i < host_runs_.size(); ++i) {
-    CC_HIP_CHECK(hipMemcpyAsync(host_ctl_ + i, host_runs_[i].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+  if (host_runs_.size() == 1) {
+    CC_HIP_CHECK(hipMemcpyAsync(host_ctl_, host_runs_[0].ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+  } else {
+    LaunchGatherCtl(runs_, Size(), ctl_table_, stream_);
+    CC_HIP_CHECK(hipGetLastError());
+    CC_HIP_CHECK(hipMemcpyAsync(host_ctl_, ctl_table_, sizeof(Ctl) * host_runs_.size(), hipMemcpyDeviceToHost,
+                                stream_));
   }
   CC_HIP_CHECK(hipStreamSynchronize(stream_));
   return host_ctl_;

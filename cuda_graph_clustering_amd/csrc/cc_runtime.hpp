@@ -5,7 +5,9 @@
 
 #include <chrono>
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,17 +36,60 @@ std::string Mebibytes(size_t bytes);
 size_t FreeDeviceBytes();
 uint32_t Words64(uint32_t n);
 
-// Device allocations released together.
+// Device memory for many short-lived blocks (a server's graphs, runs and launch tables): slabs taken from hipMalloc
+// on demand and kept until the pool is destroyed, handed out first-fit in 256-byte units. Returning a block is host
+// bookkeeping only. hipFree waits for the work of every stream on the device, so freeing through it would stall one
+// serving lane behind another lane's long kernels. The owner returns a block only after the work that used it has
+// finished on its stream. Thread-safe.
+class DevicePool {
+ public:
+  explicit DevicePool(size_t slab_bytes) : slab_bytes_(slab_bytes) {}
+  ~DevicePool();
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
+  void* Take(size_t bytes);
+  void Give(void* p);
+  size_t reserved() const;  // bytes of the slabs
+  size_t in_use() const;    // bytes of the blocks handed out
+
+ private:
+  mutable std::mutex mutex_;
+  size_t slab_bytes_;
+  std::vector<std::pair<uint8_t*, size_t>> slabs_;
+  std::map<uint8_t*, size_t> free_;           // address -> bytes; neighbours within a slab are merged
+  std::map<uint8_t*, size_t> used_;
+  size_t reserved_ = 0, in_use_ = 0;
+};
+
+// Device allocations released together: by hipFree, or back to a DevicePool when one is set before the first
+// allocation.
 class DeviceMemory {
  public:
   DeviceMemory() = default;
   DeviceMemory(const DeviceMemory&) = delete;
   DeviceMemory& operator=(const DeviceMemory&) = delete;
   ~DeviceMemory();
+  void UsePool(DevicePool* pool) { pool_ = pool; }
   void* AllocateBytes(size_t bytes);
 
  private:
+  DevicePool* pool_ = nullptr;
   std::vector<void*> blocks_;
+};
+
+// Pinned host memory that only grows; earlier buffers are kept until destruction (hipHostFree, like hipFree, waits
+// for the whole device).
+class PinnedBuffer {
+ public:
+  PinnedBuffer() = default;
+  ~PinnedBuffer();
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  void* Reserve(size_t bytes);
+
+ private:
+  std::vector<void*> buffers_;
+  size_t capacity_ = 0;
 };
 
 // A graph on the device: the bitset and the degrees, uploaded once.
@@ -58,7 +103,9 @@ struct DeviceGraph {
   DeviceMemory memory;
 };
 
-void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream);
+// With a pool, the graph's memory comes from it (and the free-memory check is left to the pool's slab allocation).
+void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hipStream_t stream,
+            DevicePool* pool = nullptr);
 void CheckParams(const cc_pbils_params& p);
 bool WidePath(uint32_t n, const cc_pbils_params& p);
 DescentPlan PlanDescent(uint32_t n, int k, const cc_pbils_params& p);
@@ -124,7 +171,7 @@ struct DescentGroup {
 // sets it takes part in, so a serving lane can add and drop runs between iterations.
 class RunState {
  public:
-  RunState(const RunSpec& spec, hipStream_t stream);
+  RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool = nullptr);
   RunState(const RunState&) = delete;
   RunState& operator=(const RunState&) = delete;
   const RunSpec& spec() const { return spec_; }
@@ -142,10 +189,12 @@ class RunState {
 
 // The launches over a set of runs: the device run table, the item tables of every launch (slots, rebuild slot
 // pairs, descent items grouped by kernel instance and ordered largest run first) and the persistent-queue
-// counters. An iteration of all runs is one fixed launch sequence.
+// counters. An iteration of all runs is one fixed launch sequence. With a pool and a pinned buffer (both kept by the
+// caller), the set allocates and frees nothing through the HIP runtime.
 class LaunchSet {
  public:
-  LaunchSet(const std::vector<RunState*>& states, hipStream_t stream);
+  LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, DevicePool* pool = nullptr,
+            PinnedBuffer* pinned = nullptr);
   ~LaunchSet();
   LaunchSet(const LaunchSet&) = delete;
   LaunchSet& operator=(const LaunchSet&) = delete;
@@ -179,8 +228,10 @@ class LaunchSet {
   std::vector<RebuildGroup> rebuild_;
   std::vector<DescentGroup> descent_;
   int32_t* queues_ = nullptr;
+  Ctl* ctl_table_ = nullptr;  // [runs] device copy of every run's Ctl, gathered for one transfer
   int max_n_ = 0;
   Ctl* host_ctl_ = nullptr;
+  bool own_host_ctl_ = false;
 };
 
 // A fixed set of runs with its launch set; iterations are replayed from one captured HIP graph.

@@ -57,6 +57,8 @@ struct Active {
 
 // Iterations a lane advances between host synchronisations while no request waits for admission.
 constexpr int kIdleChunk = 4;
+// Slabs of the server's device pool; a larger request gets a slab of its own size.
+constexpr size_t kSlabBytes = size_t(1) << 30;
 
 size_t GraphBytes(uint32_t n) {
   return static_cast<size_t>(n) * Words64(n) * 8 + kRowTailBytes + static_cast<size_t>(n) * 4;
@@ -192,6 +194,7 @@ class Server {
     (void)hipSetDevice(device_);
     Stream stream;
     const hipStream_t s = stream.get();
+    PinnedBuffer pinned;  // the launch sets' control blocks
     std::vector<Active> active;
     std::unique_ptr<LaunchSet> set;
     bool changed = false;
@@ -205,13 +208,13 @@ class Server {
         a.admitted = Clock::now();
         try {
           a.graph = std::make_unique<DeviceGraph>();
-          Upload(*a.graph, req.bits.data(), req.n, req.edges, s);
+          Upload(*a.graph, req.bits.data(), req.n, req.edges, s, &pool_);
           req.bits.clear();
           req.bits.shrink_to_fit();
           a.state = std::make_unique<RunState>(
               RunSpec{a.graph.get(), req.params, PlanDescent(req.n, req.params.k, req.params), req.params.population,
                       true},
-              s);
+              s, &pool_);
         } catch (const std::exception& e) {
           Result r;
           r.code = CC_HIP_EDEVICE;
@@ -225,8 +228,23 @@ class Server {
         active.push_back(std::move(a));
       }
       if (!fresh.empty()) {
-        LaunchSet start(fresh, s);
-        start.Start();
+        try {
+          LaunchSet start(fresh, s, &pool_, &pinned);
+          start.Start();
+          // its tables go back to the pool when `start` ends: the launches that read them must be done
+          CC_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (const std::exception& e) {
+          (void)hipStreamSynchronize(s);
+          const size_t first = active.size() - fresh.size();
+          for (size_t i = first; i < active.size(); ++i) {
+            Result r;
+            r.code = CC_HIP_EDEVICE;
+            r.error = e.what();
+            Release(active[i].request.footprint);
+            Complete(active[i].request.id, std::move(r));
+          }
+          active.resize(first);
+        }
         changed = true;
       }
       // Finished runs stay in the launch set (every kernel skips a stopped run) until new runs rebuild it anyway,
@@ -269,7 +287,7 @@ class Server {
       if (changed) {
         std::vector<RunState*> states;
         for (Active& a : active) states.push_back(a.state.get());
-        set = std::make_unique<LaunchSet>(states, s);
+        set = std::make_unique<LaunchSet>(states, s, &pool_, &pinned);
         changed = false;
       }
       // one round: one iteration of every active run, or a few while nobody waits for admission
@@ -314,6 +332,7 @@ class Server {
           Complete(a.request.id, std::move(r));
         }
       } catch (const std::exception& e) {
+        (void)hipStreamSynchronize(s);  // nothing in flight may still use the memory given back below
         for (Active& a : active) {
           if (!a.done) {
             Result r;
@@ -332,6 +351,9 @@ class Server {
 
   cc_serve_options options_;
   int device_ = 0;
+  // Every graph, run and launch table of the lanes. Blocks go back to it when a lane retires a run, after the lane's
+  // stream has finished the work that used them; the slabs are freed when the server ends.
+  DevicePool pool_{kSlabBytes};
   std::vector<Lane> lanes_;
   std::mutex mutex_;
   std::condition_variable wake_;
