@@ -167,14 +167,27 @@ __device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
   }
 }
 
-// k = 2, narrow path (n <= 32767): key = (delta + 2^15) << 16 | v << 1 | label, a 32-bit minimum.
-constexpr uint32_t kNoMove32 = ~0u;
+// Adjacency rows as 32-bit halves in the global address space, for per-lane vector loads.
+using GlobalWords32 = const __attribute__((address_space(1))) uint32_t*;
+
+// k = 2, narrow path (n <= 32767). The key of a move is (delta + 2^15) << 16 | v << 1 | label, a 32-bit minimum
+// (delta of an improving move lies in [-(n - 1), -1], so it fits).
+//
+// Thread t owns v_j = t + j T. Per slot the registers hold
+//   d_j  the delta of moving v_j to the other cluster: sv - 2 e (label 0) or 2 - sv + 2 e (label 1), with
+//        e = G1 - G0 and sv = n1 - n0 + 1;
+//   s_j  +1 (label 0), -1 (label 1), or 0 for a slot beyond n (its d_j stays 32767 and never moves);
+//   b_j  2^31 + (v_j << 1) + label, so that key_j = (d_j << 16) + b_j (mod 2^32) is the packed key and an improving
+//        move is exactly a key below 2^31.
+// When v moves out of cluster `from`, sv changes by step = (from ? -2 : 2) for everyone and e by step for each
+// neighbour, so d_j += s_j step (1 - 2 A[v][v_j]); v's own delta becomes its negation.
+constexpr uint32_t kImproving = 1u << 31;
 
 template <int T, int kVpt, bool kRowsLds>
 __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
-  static_assert(kVpt <= 32, "labels of a thread are packed into one 32-bit word");
+  static_assert(kVpt * T <= 32768, "v_j << 1 | label must fit in 16 bits");
   const int n = r.n;
   const int words = r.words;
   const int tid = threadIdx.x;
@@ -196,63 +209,87 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   const int src = src_sh;
   const int16_t* g_in = static_cast<const int16_t*>(r.pop.g) + static_cast<size_t>(src) * 2 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
-
-  int e[kVpt];
-  uint32_t label = 0;
-  uint32_t active = 0;
-#pragma unroll
-  for (int j = 0; j < kVpt; ++j) {
-    const int v = wave * kWave + lane + j * T;
-    e[j] = 0;
-    if (v < n) {
-      e[j] = static_cast<int>(g_in[n + v]) - static_cast<int>(g_in[v]);
-      label |= static_cast<uint32_t>(lab_in[v] & 1) << j;
-      active |= 1u << j;
-    }
-  }
   int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
   int64_t objective = r.pop.f[src];
 
-  // delta(0 -> 1) = sv - 2e, delta(1 -> 0) = 2 - sv + 2e with sv = n1 - n0 + 1.
-  auto scan_key = [&](int sv) {
-    int best_d = 0;
-    int best_j = -1;
+  int d[kVpt], sg[kVpt];
+  uint32_t b[kVpt];
+  {
+    const int sv = n - 2 * n0 + 1;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int d = ((label >> j) & 1u) ? 2 - sv + 2 * e[j] : sv - 2 * e[j];
-      if (((active >> j) & 1u) && d < best_d) {
-        best_d = d;
-        best_j = j;
+      const int v = tid + j * T;
+      d[j] = 32767;
+      sg[j] = 0;
+      b[j] = kImproving + (static_cast<uint32_t>(v) << 1);
+      if (v < n) {
+        const int e = static_cast<int>(g_in[n + v]) - static_cast<int>(g_in[v]);
+        const int label = lab_in[v] & 1;
+        d[j] = label ? 2 - sv + 2 * e : sv - 2 * e;
+        sg[j] = label ? -1 : 1;
+        b[j] += static_cast<uint32_t>(label);
       }
     }
-    if (best_j < 0) return kNoMove32;
-    const uint32_t v = static_cast<uint32_t>(tid + best_j * T);
-    return (static_cast<uint32_t>(best_d + 32768) << 16) | (v << 1) | ((label >> best_j) & 1u);
+  }
+  auto scan = [&]() {
+    uint32_t key = ~0u;
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) key = min(key, (static_cast<uint32_t>(d[j]) << 16) + b[j]);
+    return key;
   };
 
-  uint32_t key = scan_key(n - 2 * n0 + 1);
+  const int row_words = __builtin_amdgcn_readfirstlane(words);
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
+  const int half = lane >> 5;
+  uint32_t key = scan();
   int64_t applied = 0;
   const int64_t cap = MoveCap(objective, n);
   int parity = 0;
   while (true) {
     const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
-    if (best == kNoMove32) break;
+    if (best >= kImproving) break;
     const int delta = static_cast<int>(best >> 16) - 32768;
-    const int v = static_cast<int>((best & 0xFFFFu) >> 1);
+    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>((best & 0xFFFFu) >> 1));
     const int from = static_cast<int>(best & 1u);
+    // Row words of slot j: lanes 0-31 read the low half of the 64-bit word that holds bits base .. base + 63, lanes
+    // 32-63 the high half. All loads are issued before any is used. A slot beyond n reads the next row or the zero
+    // tail after the last row (kRowTailBytes; at most kVpt T / 8 <= 4096 bytes past the row's start) and its bit is
+    // ignored (s_j = 0).
+    uint32_t w[kVpt];
+    if constexpr (kRowsLds) {
+      const uint32_t* row = reinterpret_cast<const uint32_t*>(rows_lds + static_cast<size_t>(v) * row_words);
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) {
+        const int idx = min((wave * kWave + j * T) >> 6, row_words - 1);  // wave-uniform, stays in the row
+        w[j] = row[2 * idx + half];
+      }
+    } else {
+      const GlobalWords32 row =
+          (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + (wave * kWave >> 5) + half;
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) w[j] = row[j * (T / 32)];
+    }
     n0 += from ? 1 : -1;
     const int step = from ? -2 : 2;
-    if (v % T == tid) label ^= 1u << (v / T);
-    const uint64_t* row = (kRowsLds ? rows_lds : r.bits) + static_cast<size_t>(v) * words;
-    const int sv = n - 2 * n0 + 1;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int base = wave * kWave + j * T;  // wave-uniform
-      const uint64_t word = base < n ? row[base >> 6] : 0ull;
-      if ((word >> lane) & 1ull) e[j] += step;
+      const int bit = __builtin_amdgcn_sbfe(static_cast<int>(w[j]), lane & 31, 1);  // 0 or -1
+      const int t = __mul24(bit, 2 * step) + step;                                  // step (1 - 2 A[v][v_j])
+      d[j] += __mul24(sg[j], t);
     }
-    key = scan_key(sv);
+    // The mover: its delta becomes the reverse move's, -delta, and its label flips to 1 - from.
+    const int jv = v / T;  // uniform
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      if (j == jv && tid == v - j * T) {
+        d[j] = -delta;
+        sg[j] = from ? 1 : -1;
+        b[j] ^= 1u;
+      }
+    }
+    key = scan();
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
@@ -265,14 +302,17 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
 
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
+  const int sv = n - 2 * n0 + 1;
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
-    const int v = wave * kWave + lane + j * T;
+    const int v = tid + j * T;
     if (v < n) {
+      const int label = static_cast<int>(b[j] & 1u);
+      const int e = label ? (d[j] - 2 + sv) / 2 : (sv - d[j]) / 2;
       const int degree = r.deg[v];
-      lab_out[v] = static_cast<uint8_t>((label >> j) & 1u);
-      g_out[v] = static_cast<int16_t>((degree - e[j]) / 2);
-      g_out[n + v] = static_cast<int16_t>((degree + e[j]) / 2);
+      lab_out[v] = static_cast<uint8_t>(label);
+      g_out[v] = static_cast<int16_t>((degree - e) / 2);
+      g_out[n + v] = static_cast<int16_t>((degree + e) / 2);
     }
   }
   if (tid == 0) {
@@ -282,12 +322,6 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     r.moves[s] = applied;
   }
 }
-
-// Adjacency rows are read-only for the whole run: reading them through the constant address space lets uniform
-// row words come in with scalar loads.
-using ConstWords = const __attribute__((address_space(4))) uint64_t*;
-// The same rows as 32-bit halves in the global address space, for per-lane vector loads.
-using GlobalWords32 = const __attribute__((address_space(1))) uint32_t*;
 
 // k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt.
 //
