@@ -279,14 +279,17 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
       const int t = __mul24(bit, 2 * step) + step;                                  // step (1 - 2 A[v][v_j])
       d[j] += __mul24(sg[j], t);
     }
-    // The mover: its delta becomes the reverse move's, -delta, and its label flips to 1 - from.
-    const int jv = v / T;  // uniform
+    // The mover: its delta becomes the reverse move's, -delta, and its label flips to 1 - from. Only the wave that
+    // owns it takes the (uniform) branch.
+    if (wave == ((v & (T - 1)) >> 6)) {
+      const int jv = v / T;  // uniform
 #pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      if (j == jv && tid == v - j * T) {
-        d[j] = -delta;
-        sg[j] = from ? 1 : -1;
-        b[j] ^= 1u;
+      for (int j = 0; j < kVpt; ++j) {
+        if (j == jv && tid == v - j * T) {
+          d[j] = -delta;
+          sg[j] = from ? 1 : -1;
+          b[j] ^= 1u;
+        }
       }
     }
     key = scan();
