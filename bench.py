@@ -8,8 +8,9 @@ graph-seed 2; k=5, pop 128, gww 0.1, seed 42, early stop off. The graph is uploa
 the timed region is one solve of K iterations (its allocation and initial rebuild included). Prints one JSON line.
 
 value       PBILS iterations per second (whole job, 1 GPU)
-roofline    the descent kernel (the dominant one): algorithmic bytes per launch / its mean launch time, measured
-            with HIP events on the library's stream over the timed region
+roofline    the descent kernel (the dominant one): algorithmic global-memory bytes per launch / its mean launch
+            time, measured with HIP events on the library's stream over the timed region; for the register kernels
+            also the VALU figure (algorithmic lane operations per move), the roof that binds them
 cpu_baseline  the reference CPU solver (built out of tree into oracle/_ref) on this host's cores, on a bounded
             sample of the same workload; bit_exact reports that the GPU reproduces its result on that sample.
 """
@@ -29,12 +30,14 @@ sys.path.insert(0, ROOT)
 # accepted move). A timed solve of the same length has the same trajectory, so its launches move the same bytes;
 # other lengths are scaled by their accepted moves. Commands and counter values: profiles/descent_*.txt.
 MEASURED_TRAFFIC = {
-    "c2": (200, 433203205, 853.8),
+    "c2": (200, 438537922, 864.3),
     "c5": (2, 195321610816, 30241.7),
 }
 
 METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
+# int32 VALU lanes: 256 CUs x 64 lanes x 2.4 GHz
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 CONFIGS = {
     "c1": dict(workload="ER G(n=1000, p=0.5) graph-seed 1; k=2, pop=128, gww=0.1, seed=42, early stop off",
@@ -63,17 +66,20 @@ def make_graph(cga, gen):
     return cga.sbm(gen[1], gen[2], gen[3], gen[4], gen[5])
 
 
+REGISTER_DESCENTS = ("registers", "k2-registers", "generic-lds")
+
+
 def descent_bytes_per_move(n, k, descent, mean_degree):
-    """Algorithmic bytes one accepted move streams: the slot's state scanned once plus the adjacency row of the
-    moved vertex (and, for the packed wide kernels, the write-back of the neighbours' words). The state is G and
-    the labels (k n |G| + n) for the generic and register kernels, e and the label bit (4 n) for the k = 2 register
-    kernel, and one packed word per vertex (4 n for k = 2, 8 n for k = 3) for the wide kernels. The chunk-summary
-    kernel reads the row and re-reads only the 64-vertex chunks holding the moved vertex and its neighbours (at most
-    mean degree + 1 chunks of 512 bytes, written back)."""
+    """Algorithmic global-memory bytes of one accepted move: the adjacency row of the moved vertex, plus what the
+    kernel keeps in memory rather than on chip. The register kernels (and the generic kernel with G in LDS) hold
+    the individual's state on chip, so a move reads only the row. The generic global kernel scans G and the labels
+    (k n |G| + n) in memory every move. The wide kernels stream one packed word per vertex (4 n for k = 2, 8 n for
+    k = 3) and write back the neighbours' words; the chunk-summary kernel re-reads only the 64-vertex chunks that
+    hold the moved vertex and its neighbours (at most mean degree + 1 chunks of 512 bytes, written back)."""
     words = max(1, (n + 63) // 64)
     row = 8 * words
-    if descent == "k2-registers":
-        return 4 * n + row
+    if descent in REGISTER_DESCENTS:
+        return row
     if descent == "k2-wide":
         return 4 * n + row + 4 * mean_degree
     if descent == "k3-wide":
@@ -82,6 +88,13 @@ def descent_bytes_per_move(n, k, descent, mean_degree):
         return row + 2 * 512 * min(mean_degree + 1, words)
     gbytes = 2 if n <= 32767 else 4
     return k * n * gbytes + n + row
+
+
+def descent_state_bytes(n, k):
+    """Bytes of one individual's state that a descent reads at its start and writes at its end: G (k n |G|) and
+    the labels (n), each way."""
+    gbytes = 2 if n <= 32767 else 4
+    return 2 * (k * n * gbytes + n)
 
 
 def reference_solver():
@@ -371,7 +384,9 @@ def main():
     launches = max(1, res["search_launches"])
     mean_launch_s = res["search_ms"] / 1e3 / launches
     moves_per_launch = res["accepted_moves"] / launches
-    bytes_per_launch = moves_per_launch * descent_bytes_per_move(n, k, res["descent_name"], 2.0 * graph.edges / n)
+    descent = res["descent_name"]
+    bytes_per_launch = (moves_per_launch * descent_bytes_per_move(n, k, descent, 2.0 * graph.edges / n) +
+                        cfg["population"] * descent_state_bytes(n, k))
     traffic = None
     if args.config in MEASURED_TRAFFIC:
         t_steps, t_launch, t_move = MEASURED_TRAFFIC[args.config]
@@ -410,6 +425,15 @@ def main():
             "mean_launch_ms": round(1e3 * mean_launch_s, 4),
         },
     }
+    if descent in REGISTER_DESCENTS and mean_launch_s > 0:
+        # The state never leaves the CU, so HBM is far from binding; the roof is VALU issue. Algorithmic lane
+        # operations per move: 7 (k - 1) n (scan every (vertex, target) pair and update the neighbours).
+        ops = 7 * (k - 1) * n
+        valu = moves_per_launch * ops / mean_launch_s
+        out["roofline"]["valu"] = {
+            "achieved": round(valu / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2), "unit": "Tlane-op/s",
+            "frac": round(valu / VALU_PEAK_LANE_OPS, 4), "ops_per_move": ops,
+            "cus_used": min(cfg["population"], 256)}
     if sample is not None:
         m = sample["iterations"]
         cpu_rate = 1.0 / sample["cpu_iteration_s"]

@@ -368,13 +368,34 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * K;
 
-  int sz[K];
-  int kc[K];  // (s_c << 12) + c
+  // Cluster sizes (< 2^15 on the narrow path) as 16-bit fields, four per 64-bit word, wave-uniform: a move updates
+  // them with scalar shifts and adds.
+  constexpr int kSizeWords = (K + 3) / 4;
+  uint64_t szp[kSizeWords];
+#pragma unroll
+  for (int i = 0; i < kSizeWords; ++i) szp[i] = 0;
 #pragma unroll
   for (int c = 0; c < K; ++c) {
-    sz[c] = __builtin_amdgcn_readfirstlane(sizes_in[c]);
-    kc[c] = (sz[c] << 12) + c;
+    szp[c >> 2] |= static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(sizes_in[c])) << (16 * (c & 3));
   }
+  auto size_of = [&](int c) {  // c known at compile time
+    return static_cast<int>((szp[c >> 2] >> (16 * (c & 3))) & 0xFFFFu);
+  };
+  auto size_add = [&](int c, int64_t d) {  // c uniform
+    const uint64_t inc = static_cast<uint64_t>(d) << (16 * (c & 3));
+    if constexpr (kSizeWords == 1) {
+      szp[0] += inc;
+    } else {
+      if (c < 4) {
+        szp[0] += inc;
+      } else {
+        szp[1] += inc;
+      }
+    }
+  };
+  int kc[K];  // (s_c << 12) + c at the start
+#pragma unroll
+  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
 
   int y[kVpt][K];
   int qa[kVpt];
@@ -396,7 +417,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
         y[j][c] = (g << 13) - (j << 6) - (a << 3) - kc[c];
         if (c == a) {
           ga = g;
-          sa = sz[c];
+          sa = size_of(c);
         }
       }
       qa[j] = (sa << 12) - (ga << 13);
@@ -445,21 +466,13 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
         (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + (wave * kWave >> 5) + half;
     uint32_t words[kVpt];
 #pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      const int base = wave * kWave + j * T;  // wave-uniform
-      const uint32_t w = row[j * (T / 32)];
-      words[j] = base < n ? w : 0u;
-    }
+    for (int j = 0; j < kVpt; ++j) words[j] = row[j * (T / 32)];
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
     const int from = (info >> 3) & 7;
     const int to = info & 7;
     parity ^= 1;
-#pragma unroll
-    for (int c = 0; c < K; ++c) {  // uniform: scalar branches
-      if (c == to) ++sz[c];
-      if (c == from) --sz[c];
-      kc[c] = (sz[c] << 12) + c;
-    }
+    size_add(to, 1);
+    size_add(from, -1);
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
     int m[kVpt];
@@ -481,20 +494,32 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
         for (int j = 0; j < kVpt; ++j) y[j][c] -= m[j];
       }
     }
-    // the moved vertex: new label bits in its y, its own term from its (unchanged) G[to] and the new s_to
+    // The moved vertex: new label bits in its y, and its own term qa = (s_to << 12) - (G_to << 13), which is
+    // -(y[to] + (j << 6) + (to << 3) + to) once y carries the new sizes and label.
     const int mv_j = v / T;
     if (tid == v - mv_j * T) {
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
         if (j == mv_j) {
           lab4[j] = to * 4;
-          int g_to13 = 0;
+          int y_to = 0;
 #pragma unroll
           for (int c = 0; c < K; ++c) {
             y[j][c] -= (to - from) << 3;
-            if (c == to) g_to13 = y[j][c] + kc[c] + (j << 6) + (to << 3);
+            if (c == to) y_to = y[j][c];
           }
-          qa[j] = (sz[to] << 12) - g_to13;
+          qa[j] = -(y_to + (j << 6) + (to << 3) + to);
+        }
+      }
+    }
+    // Row words of slots beyond n are not masked: a padding lane's y (kPadY) drifts by at most 4096 per move, so
+    // it is reset every 4096 moves, long before it could come near a real entry (|y| < 2^29).
+    if ((applied & 4095) == 4095) {
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) {
+        if (tid + j * T >= n) {
+#pragma unroll
+          for (int c = 0; c < K; ++c) y[j][c] = kPadY;
         }
       }
     }
@@ -512,6 +537,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * K * n;
 #pragma unroll
+  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
+#pragma unroll
   for (int j = 0; j < kVpt; ++j) {
     const int v = tid + j * T;
     if (v < n) {
@@ -526,7 +553,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   if (tid < K) {
     int size = 0;
 #pragma unroll
-    for (int c = 0; c < K; ++c) size = tid == c ? sz[c] : size;
+    for (int c = 0; c < K; ++c) size = tid == c ? size_of(c) : size;
     r.next.sizes[static_cast<size_t>(s) * K + tid] = size;
   }
   if (tid == 0) {

@@ -35,8 +35,11 @@ __device__ __forceinline__ uint64_t DppMinStep(uint64_t x) {
 
 template <int kCtrl>
 __device__ __forceinline__ uint32_t DppMinStep(uint32_t x) {
-  const uint32_t other = DppMove<kCtrl>(x);
-  return other < x ? other : x;
+  // Every lane's source is valid for these four patterns, so bound_ctrl may be set; with it the compiler folds the
+  // move into the minimum (one v_min_u32_dpp per step).
+  const uint32_t other =
+      static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(x), kCtrl, 0xF, 0xF, true));
+  return min(other, x);
 }
 
 // Minimum over each row of 16 lanes, present in every lane of the row.
