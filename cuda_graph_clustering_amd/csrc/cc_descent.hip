@@ -173,14 +173,14 @@ using GlobalWords32 = const __attribute__((address_space(1))) uint32_t*;
 // k = 2, narrow path (n <= 32767). The key of a move is (delta + 2^15) << 16 | v << 1 | label, a 32-bit minimum
 // (delta of an improving move lies in [-(n - 1), -1], so it fits).
 //
-// Thread t owns v_j = t + j T. Per slot the registers hold
-//   d_j  the delta of moving v_j to the other cluster: sv - 2 e (label 0) or 2 - sv + 2 e (label 1), with
-//        e = G1 - G0 and sv = n1 - n0 + 1;
-//   s_j  +1 (label 0), -1 (label 1), or 0 for a slot beyond n (its d_j stays 32767 and never moves);
-//   b_j  2^31 + (v_j << 1) + label, so that key_j = (d_j << 16) + b_j (mod 2^32) is the packed key and an improving
-//        move is exactly a key below 2^31.
+// Thread t owns v_j = t + j T. Per slot two VGPRs hold
+//   key_j  ((d_j + 2^15) << 16) | (v_j << 1) | label, where d_j is the delta of moving v_j to the other cluster:
+//          sv - 2 e (label 0) or 2 - sv + 2 e (label 1), with e = G1 - G0 and sv = n1 - n0 + 1. |d_j| <= n - 1,
+//          so key_j is the packed key of v_j's move, and the move improves exactly when key_j < 2^31;
+//   sg_j   +2^16 (label 0), -2^16 (label 1), or 0 for a slot beyond n (whose key stays at d = 32767).
 // When v moves out of cluster `from`, sv changes by step = (from ? -2 : 2) for everyone and e by step for each
-// neighbour, so d_j += s_j step (1 - 2 A[v][v_j]); v's own delta becomes its negation.
+// neighbour, so d_j += +-step (1 - 2 A[v][v_j]), i.e. key_j += sg_j step (1 - 2 A[v][v_j]); v's own key becomes
+// that of the reverse move, -delta, with the flipped label.
 constexpr uint32_t kImproving = 1u << 31;
 
 template <int T, int kVpt, bool kRowsLds>
@@ -212,42 +212,42 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
   int64_t objective = r.pop.f[src];
 
-  int d[kVpt], sg[kVpt];
-  uint32_t b[kVpt];
+  uint32_t key[kVpt];
+  int sg[kVpt];
   {
+    // Branch-free: a slot beyond n loads vertex n - 1 and discards it (branches around writes to the slot arrays
+    // make the compiler copy whole arrays between blocks).
     const int sv = n - 2 * n0 + 1;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
       const int v = tid + j * T;
-      d[j] = 32767;
-      sg[j] = 0;
-      b[j] = kImproving + (static_cast<uint32_t>(v) << 1);
-      if (v < n) {
-        const int e = static_cast<int>(g_in[n + v]) - static_cast<int>(g_in[v]);
-        const int label = lab_in[v] & 1;
-        d[j] = label ? 2 - sv + 2 * e : sv - 2 * e;
-        sg[j] = label ? -1 : 1;
-        b[j] += static_cast<uint32_t>(label);
-      }
+      const bool real = v < n;
+      const int vc = real ? v : n - 1;
+      const int e = static_cast<int>(g_in[n + vc]) - static_cast<int>(g_in[vc]);
+      const int label = real ? (lab_in[vc] & 1) : 0;
+      const int d = real ? (label ? 2 - sv + 2 * e : sv - 2 * e) : 32767;
+      sg[j] = real ? (label ? -65536 : 65536) : 0;
+      key[j] = (static_cast<uint32_t>(d + 32768) << 16) | (static_cast<uint32_t>(v) << 1) |
+               static_cast<uint32_t>(label);
     }
   }
   auto scan = [&]() {
-    uint32_t key = ~0u;
+    uint32_t m = ~0u;
 #pragma unroll
-    for (int j = 0; j < kVpt; ++j) key = min(key, (static_cast<uint32_t>(d[j]) << 16) + b[j]);
-    return key;
+    for (int j = 0; j < kVpt; ++j) m = min(m, key[j]);
+    return m;
   };
 
   const int row_words = __builtin_amdgcn_readfirstlane(words);
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int half = lane >> 5;
-  uint32_t key = scan();
+  uint32_t mine = scan();
   int64_t applied = 0;
   const int64_t cap = MoveCap(objective, n);
   int parity = 0;
   while (true) {
-    const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
+    const uint32_t best = BlockMin<kWaves>(mine, partial[parity]);
     parity ^= 1;
     if (best >= kImproving) break;
     const int delta = static_cast<int>(best >> 16) - 32768;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     // Row words of slot j: lanes 0-31 read the low half of the 64-bit word that holds bits base .. base + 63, lanes
     // 32-63 the high half. All loads are issued before any is used. A slot beyond n reads the next row or the zero
     // tail after the last row (kRowTailBytes; at most kVpt T / 8 <= 4096 bytes past the row's start) and its bit is
-    // ignored (s_j = 0).
+    // ignored (sg_j = 0).
     uint32_t w[kVpt];
     if constexpr (kRowsLds) {
       const uint32_t* row = reinterpret_cast<const uint32_t*>(rows_lds + static_cast<size_t>(v) * row_words);
@@ -277,22 +277,23 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     for (int j = 0; j < kVpt; ++j) {
       const int bit = __builtin_amdgcn_sbfe(static_cast<int>(w[j]), lane & 31, 1);  // 0 or -1
       const int t = __mul24(bit, 2 * step) + step;                                  // step (1 - 2 A[v][v_j])
-      d[j] += __mul24(sg[j], t);
+      key[j] += static_cast<uint32_t>(__mul24(sg[j], t));
     }
-    // The mover: its delta becomes the reverse move's, -delta, and its label flips to 1 - from. Only the wave that
-    // owns it takes the (uniform) branch.
+    // The mover: its key becomes the reverse move's (delta -> -delta, label 1 - from). Only the wave that owns it
+    // takes the (uniform) branch.
     if (wave == ((v & (T - 1)) >> 6)) {
       const int jv = v / T;  // uniform
+      const uint32_t flipped = (static_cast<uint32_t>(32768 - delta) << 16) | (static_cast<uint32_t>(v) << 1) |
+                               static_cast<uint32_t>(1 - from);
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
         if (j == jv && tid == v - j * T) {
-          d[j] = -delta;
-          sg[j] = from ? 1 : -1;
-          b[j] ^= 1u;
+          key[j] = flipped;
+          sg[j] = from ? 65536 : -65536;
         }
       }
     }
-    key = scan();
+    mine = scan();
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
@@ -310,8 +311,9 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   for (int j = 0; j < kVpt; ++j) {
     const int v = tid + j * T;
     if (v < n) {
-      const int label = static_cast<int>(b[j] & 1u);
-      const int e = label ? (d[j] - 2 + sv) / 2 : (sv - d[j]) / 2;
+      const int label = static_cast<int>(key[j] & 1u);
+      const int d = static_cast<int>(key[j] >> 16) - 32768;
+      const int e = label ? (d - 2 + sv) / 2 : (sv - d) / 2;
       const int degree = r.deg[v];
       lab_out[v] = static_cast<uint8_t>(label);
       g_out[v] = static_cast<int16_t>((degree - e) / 2);
@@ -400,28 +402,24 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   int y[kVpt][K];
   int qa[kVpt];
   int lab4[kVpt];  // label * 4: the ds_bpermute address of the lane table
+  // Branch-free: a padding slot (v >= n) loads vertex n - 1 and keeps its padding values (branches around writes
+  // to the slot arrays make the compiler copy whole arrays between blocks).
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
     const int v = tid + j * T;
-    qa[j] = 0;
-    lab4[j] = kPadLabel * 4;
+    const bool real = v < n;
+    const int vc = real ? v : n - 1;
+    const int a = lab_in[vc];
+    int ga = 0, sa = 0;
 #pragma unroll
-    for (int c = 0; c < K; ++c) y[j][c] = kPadY;
-    if (v < n) {
-      const int a = lab_in[v];
-      lab4[j] = a * 4;
-      int ga = 0, sa = 0;
-#pragma unroll
-      for (int c = 0; c < K; ++c) {
-        const int g = g_in[static_cast<size_t>(c) * n + v];
-        y[j][c] = (g << 13) - (j << 6) - (a << 3) - kc[c];
-        if (c == a) {
-          ga = g;
-          sa = size_of(c);
-        }
-      }
-      qa[j] = (sa << 12) - (ga << 13);
+    for (int c = 0; c < K; ++c) {
+      const int g = g_in[static_cast<size_t>(c) * n + vc];
+      y[j][c] = real ? (g << 13) - (j << 6) - (a << 3) - kc[c] : kPadY;
+      ga = c == a ? g : ga;
+      sa = c == a ? size_of(c) : sa;
     }
+    qa[j] = real ? (sa << 12) - (ga << 13) : 0;
+    lab4[j] = (real ? a : kPadLabel) * 4;
   }
   const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;

@@ -249,3 +249,15 @@ def test_wide_chunk_kernel_with_several_chunks_per_thread(cga, orc, cpu_threads,
         cga.tune_clear()
     check_same(got, want, f"chunk kernel {threads} threads")
     assert (got["descent_name"], got["threads"]) == ("k3-chunk", threads)
+
+
+def test_register_kernel_descent_longer_than_padding_reset(cga, orc, cpu_threads):
+    """Descents of more than 4096 moves in the register kernel with padding slots (n = 12000 on 12 x 1024 slots):
+    padding lanes take unmasked row words and are reset every 4096 moves, so a first descent from random labels
+    must still follow the oracle exactly."""
+    g = cga.sbm(12000, 3, 0.4, 0.05, 21)
+    kw = dict(k=3, population=2, iterations=1, early_stop=1, seed=4, gww=0.1)
+    got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    check_same(got, want, "register kernel, long descent")
+    assert got["descent_name"] == "registers"
+    assert got["accepted_moves"] > 2 * 4096, "the descents must pass the padding reset"
