@@ -140,6 +140,8 @@ ABI_SYMBOLS = [
     "cc_graph_chung_lu",
     "cc_objective_direct",
     "cc_objective_bitwise",
+    "cc_graph_load_tags",
+    "cc_sample_without_replacement",
     "cc_hip_device_info",
     "cc_hip_autotune",
     "cc_hip_tune_clear",
@@ -193,6 +195,12 @@ def _load():
     for name in ("cc_objective_direct", "cc_objective_bitwise"):
         getattr(lib, name).argtypes = [_u64p, ctypes.c_uint32, _i32p]
         getattr(lib, name).restype = ctypes.c_int64
+    lib.cc_graph_load_tags.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_uint32,
+                                       ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_graph_load_tags.restype = ctypes.c_int
+    lib.cc_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                                  ctypes.POINTER(ctypes.c_uint32)]
     lib.cc_hip_device_info.argtypes = [ctypes.POINTER(DeviceInfo), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_hip_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32,
                                     ctypes.POINTER(TuneCandidate), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
@@ -246,6 +254,29 @@ def from_bits(bits, n):
     bits = np.ascontiguousarray(bits, dtype=np.uint64).reshape(n, words64(n))
     edges = int(np.unpackbits(bits.view(np.uint8)).sum(dtype=np.int64)) // 2
     return Graph(bits, n, edges)
+
+
+def load_tags(path, similarity="jaccard", threshold=0.5, n=0, seed=1):
+    """Graph::LoadTags: the tag-similarity graph of a {"key": ["tag", ...]} JSON file over n objects sampled by
+    seed (0: all), an edge when the similarity of two tag sets is >= threshold."""
+    size = ctypes.c_uint32(0)
+    edges = ctypes.c_uint64(0)
+    err = ctypes.create_string_buffer(512)
+    args = (os.fsencode(path), similarity.encode(), float(threshold), int(n), int(seed))
+    rc = lib.cc_graph_load_tags(*args, None, 0, ctypes.byref(size), ctypes.byref(edges), err, 512)
+    if rc not in (0, ENOMEM):
+        _check(rc, err)
+    bits = np.zeros((size.value, words64(size.value)), dtype=np.uint64)
+    _check(lib.cc_graph_load_tags(*args, bits.ctypes.data_as(_u64p), bits.size, ctypes.byref(size),
+                                  ctypes.byref(edges), err, 512), err)
+    return Graph(bits, size.value, edges.value)
+
+
+def sample_without_replacement(total, n, seed):
+    """SampleWithoutReplacement: the objects LoadTags takes, in vertex order."""
+    out = np.zeros(max(total, 1), dtype=np.uint32)
+    lib.cc_sample_without_replacement(total, n, seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return out[: (total if n == 0 or n >= total else n)]
 
 
 def objective_bitwise(graph, labels):

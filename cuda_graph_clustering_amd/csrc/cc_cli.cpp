@@ -38,6 +38,8 @@ struct Options {
   double gamma = 2.5;
   double mean_degree = 64.0;
   std::string graph_path, graph_json_path, edges_path, save_graph_path, save_graph_json_path, save_edges_path;
+  std::string tags_path, similarity = "jaccard";
+  double threshold = 0.5;
   cc::PbilsParams params;
   int algorithm = kPbils;
   int runs = 1;
@@ -100,6 +102,12 @@ std::vector<Flag> Flags(Options& o) {
       Option("--graph", "PATH", "load a 0/1 matrix instead of generating", o.graph_path),
       Option("--graph-json", "PATH", "load the \"graph\" block of a baseline result file", o.graph_json_path),
       Option("--edges", "PATH", "load an edge list (SNAP, CSV): vertex = rank of its id", o.edges_path),
+      Option("--tags", "PATH",
+             "tag-similarity graph of a {\"key\": [tags]} JSON file: --n objects sampled by --graph-seed "
+             "(0: all), joined when the similarity of their tags is >= --threshold",
+             o.tags_path),
+      Option("--similarity", "M", "measure for --tags: jaccard, cosine, dice or overlap", o.similarity),
+      Option("--threshold", "T", "threshold for --tags", o.threshold),
       Option("--save-graph", "PATH", "save the instance as a 0/1 matrix", o.save_graph_path),
       Option("--save-graph-json", "PATH", "save the instance in the baseline JSON format", o.save_graph_json_path),
       Option("--save-edges", "PATH", "save the instance as an edge list", o.save_edges_path),
@@ -174,6 +182,7 @@ bool Parse(int argc, char** argv, Options& options) {
 }
 
 cc::Graph LoadInstance(const Options& o) {
+  if (!o.tags_path.empty()) return cc::Graph::LoadTags(o.tags_path, o.similarity, o.threshold, o.n, o.graph_seed);
   if (!o.graph_json_path.empty()) return cc::Graph::LoadBaselineJson(o.graph_json_path);
   if (!o.edges_path.empty()) return cc::Graph::LoadEdgeList(o.edges_path);
   if (!o.graph_path.empty()) return cc::Graph::LoadMatrix(o.graph_path);

@@ -9,10 +9,12 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "cc_gpu.hpp"
@@ -281,6 +283,191 @@ void Graph::SaveEdgeList(const std::string& path) const {
   }
 }
 
+// ---- tag-similarity graph ----------------------------------------------------------------------------------------
+
+namespace {
+
+// {"key": ["tag", ...], ...} -> per object, in file order, its tag ids sorted and without repeats. Only this shape
+// is read (the dataset files have exactly it); inside a string a backslash takes the next character as it is, which
+// is enough to tell tags apart. Keys are not used: an object's vertex is its position in the file.
+std::vector<std::vector<int>> ParseTagSets(const std::string& path) {
+  const std::string text = ReadWholeFile(path);
+  size_t pos = 0;
+  auto fail = [&](const char* what) -> void {
+    throw std::runtime_error(std::string("malformed tags json (") + what + ") at byte " + std::to_string(pos) +
+                             " of " + path);
+  };
+  auto peek = [&]() {
+    while (pos < text.size() && std::isspace(static_cast<unsigned char>(text[pos]))) ++pos;
+    return pos < text.size() ? text[pos] : '\0';
+  };
+  auto take = [&](char ch) {
+    if (peek() != ch) fail("unexpected character");
+    ++pos;
+  };
+  std::string token;
+  auto string = [&]() {
+    take('"');
+    token.clear();
+    for (; pos < text.size() && text[pos] != '"'; ++pos) {
+      if (text[pos] == '\\' && ++pos >= text.size()) break;
+      token += text[pos];
+    }
+    if (pos >= text.size()) fail("unterminated string");
+    ++pos;
+  };
+  std::unordered_map<std::string, int> ids;
+  std::vector<std::vector<int>> sets;
+  take('{');
+  if (peek() == '}') return sets;
+  while (true) {
+    string();
+    take(':');
+    take('[');
+    std::vector<int> tags;
+    while (peek() != ']') {
+      if (!tags.empty()) take(',');
+      string();
+      tags.push_back(ids.emplace(token, static_cast<int>(ids.size())).first->second);
+    }
+    take(']');
+    std::sort(tags.begin(), tags.end());
+    tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+    sets.push_back(std::move(tags));
+    if (peek() == '}') break;
+    take(',');
+  }
+  return sets;
+}
+
+enum class TagMeasure { kJaccard, kCosine, kDice, kOverlap };
+
+TagMeasure ParseTagMeasure(const std::string& name) {
+  if (name == "jaccard") return TagMeasure::kJaccard;
+  if (name == "cosine") return TagMeasure::kCosine;
+  if (name == "dice") return TagMeasure::kDice;
+  if (name == "overlap") return TagMeasure::kOverlap;
+  throw std::runtime_error("unknown similarity '" + name + "', expected jaccard, cosine, dice or overlap");
+}
+
+// The baseline's formulas in double precision, evaluated in its order (reference src/cc_graph.cpp:108-118), so the
+// comparison with the threshold agrees with it to the last bit; 0/0 of empty sets is NaN and never reaches it.
+bool TagsJoined(TagMeasure m, size_t common, size_t a, size_t b, double threshold) {
+  const double inter = static_cast<double>(common);
+  double s = 0.0;
+  switch (m) {
+    case TagMeasure::kJaccard: s = inter / static_cast<double>(a + b - common); break;
+    case TagMeasure::kCosine: s = inter / std::sqrt(static_cast<double>(a) * static_cast<double>(b)); break;
+    case TagMeasure::kDice: s = 2.0 * inter / (static_cast<double>(a) + static_cast<double>(b)); break;
+    case TagMeasure::kOverlap: s = inter / std::min(static_cast<double>(a), static_cast<double>(b)); break;
+  }
+  return s >= threshold;
+}
+
+}  // namespace
+
+std::vector<unsigned> SampleWithoutReplacement(unsigned total, unsigned n, uint64_t seed) {
+  std::vector<unsigned> order(total);
+  for (unsigned i = 0; i < total; ++i) order[i] = i;
+  if (n == 0 || n >= total) return order;
+  uint64_t state = StreamSeed(seed, kSampleStream, 0);
+  for (unsigned i = 0; i < n; ++i) std::swap(order[i], order[i + RandBelow(state, total - i)]);
+  order.resize(n);
+  return order;
+}
+
+// Vertices with the same tag set have the same row (but for their own diagonal bit), so the rows are built once per
+// distinct set (a class). The classes that share at least one tag with a class are found through per-tag lists of
+// classes, with their common-tag counts, instead of by comparing every pair of classes; pairs without a common tag
+// are joined only if the measure of an empty intersection passes the threshold (threshold <= 0), which is decided
+// once per pair of set sizes. A row is then the union of the member lists of its joined classes.
+Graph Graph::LoadTags(const std::string& path, const std::string& similarity, double threshold, unsigned n,
+                      uint64_t seed) {
+  const TagMeasure measure = ParseTagMeasure(similarity);
+  const std::vector<std::vector<int>> sets = ParseTagSets(path);
+  if (sets.empty()) throw std::runtime_error("no objects in " + path);
+  const std::vector<unsigned> objects = SampleWithoutReplacement(static_cast<unsigned>(sets.size()), n, seed);
+  const unsigned size = static_cast<unsigned>(objects.size());
+
+  // classes, their members (vertices in increasing order) and the per-tag class lists
+  std::map<std::vector<int>, int> class_of_set;
+  std::vector<const std::vector<int>*> class_tags;
+  std::vector<std::vector<unsigned>> members;
+  for (unsigned v = 0; v < size; ++v) {
+    const auto found = class_of_set.emplace(sets[objects[v]], static_cast<int>(class_tags.size()));
+    if (found.second) {
+      class_tags.push_back(&found.first->first);
+      members.emplace_back();
+    }
+    members[found.first->second].push_back(v);
+  }
+  const int classes = static_cast<int>(class_tags.size());
+  int tag_count = 0;
+  for (const auto* tags : class_tags) tag_count = std::max(tag_count, tags->empty() ? 0 : tags->back() + 1);
+  std::vector<std::vector<int>> classes_of_tag(tag_count);
+  for (int c = 0; c < classes; ++c) {
+    for (int t : *class_tags[c]) classes_of_tag[t].push_back(c);
+  }
+  // empty intersections: joined iff the measure of (0, a, b) passes, for the distinct set sizes a, b
+  std::vector<size_t> sizes;
+  for (const auto* tags : class_tags) sizes.push_back(tags->size());
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  bool any_disjoint = false;
+  for (size_t a : sizes) {
+    for (size_t b : sizes) any_disjoint = any_disjoint || TagsJoined(measure, 0, a, b, threshold);
+  }
+
+  Graph g(size);
+  const uint32_t words = g.words_per_row_;
+  std::atomic<uint64_t> degree_sum{0};
+  std::atomic<int> next{0};
+  const int workers = std::max(1, std::min(HardwareThreads(), classes));
+  ParallelFor(workers, workers, [&](int) {
+    std::vector<uint32_t> common(classes, 0);
+    std::vector<int> touched;
+    std::vector<Word> row(words);
+    for (int c = next.fetch_add(1); c < classes; c = next.fetch_add(1)) {
+      const size_t a = class_tags[c]->size();
+      touched.clear();
+      for (int t : *class_tags[c]) {
+        for (int other : classes_of_tag[t]) {
+          if (common[other]++ == 0) touched.push_back(other);
+        }
+      }
+      std::fill(row.begin(), row.end(), Word(0));
+      auto add_members = [&](int other) {
+        for (unsigned u : members[other]) row[u / kWordBits] |= Word(1) << (u % kWordBits);
+      };
+      if (any_disjoint) {
+        for (int other = 0; other < classes; ++other) {
+          if (TagsJoined(measure, common[other], a, class_tags[other]->size(), threshold)) add_members(other);
+        }
+        for (int other : touched) common[other] = 0;
+      } else {
+        for (int other : touched) {
+          if (TagsJoined(measure, common[other], a, class_tags[other]->size(), threshold)) add_members(other);
+          common[other] = 0;
+        }
+      }
+      uint64_t ones = 0;
+      for (Word w : row) ones += static_cast<uint64_t>(__builtin_popcountll(w));
+      // the class is joined with itself iff its own bits are set (a set's similarity to itself is 1, or NaN if
+      // empty); a vertex has no loop
+      const unsigned first = members[c][0];
+      const bool self = (row[first / kWordBits] >> (first % kWordBits)) & 1u;
+      for (unsigned v : members[c]) {
+        Word* dst = g.bits_.data() + static_cast<size_t>(v) * words;
+        std::copy(row.begin(), row.end(), dst);
+        dst[v / kWordBits] &= ~(Word(1) << (v % kWordBits));
+      }
+      degree_sum += (ones - (self ? 1 : 0)) * members[c].size();
+    }
+  });
+  g.edges_ = degree_sum.load() / 2;
+  return g;
+}
+
 // ---- recounts and GWW helpers -----------------------------------------------------------------------------------
 
 long long ObjectiveDirect(const Graph& graph, const std::vector<int>& labels) {
@@ -471,6 +658,31 @@ uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_
                            int32_t threads) {
   cc::GeneratorChungLu(n, gamma, mean_degree, seed, bits, threads);
   return cc::CountEdges(bits, static_cast<size_t>(n) * cc::WordsFor(n));
+}
+
+int cc_graph_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
+                       uint64_t* bits, uint64_t capacity_words, uint32_t* n_out, uint64_t* edges_out, char* err,
+                       size_t err_len) {
+  try {
+    const cc::Graph g = cc::Graph::LoadTags(path, similarity, threshold, n, seed);
+    *n_out = g.Size();
+    *edges_out = g.EdgeCount();
+    const uint64_t need = static_cast<uint64_t>(g.Size()) * g.WordsPerRow();
+    if (capacity_words < need) {
+      if (err && err_len) std::snprintf(err, err_len, "bits needs %llu words", static_cast<unsigned long long>(need));
+      return CC_HIP_ENOMEM;
+    }
+    std::copy(g.Bits().begin(), g.Bits().end(), bits);
+    return CC_HIP_OK;
+  } catch (const std::exception& e) {
+    if (err && err_len) std::snprintf(err, err_len, "%s", e.what());
+    return CC_HIP_EINVAL;
+  }
+}
+
+void cc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
+  const std::vector<unsigned> order = cc::SampleWithoutReplacement(total, n, seed);
+  std::copy(order.begin(), order.end(), out);
 }
 
 int64_t cc_objective_direct(const uint64_t* bits, uint32_t n, const int32_t* labels) {

@@ -50,6 +50,12 @@ class Graph {
   // SNAP / CSV edge list: vertex = rank of its id among all ids (reference src/cc_graph.cpp:270-306).
   static Graph LoadEdgeList(const std::string& path);
   void SaveEdgeList(const std::string& path) const;
+  // Tag-similarity graph (reference src/cc_graph.cpp:311-371): a JSON object {"key": ["tag", ...], ...}; the
+  // vertices are SampleWithoutReplacement(objects, n, seed) (n = 0: all objects, in file order) and u, v are
+  // joined iff the similarity of their tag sets (jaccard, cosine, dice or overlap, in double precision) is at least
+  // threshold. No loops.
+  static Graph LoadTags(const std::string& path, const std::string& similarity, double threshold, unsigned n,
+                        uint64_t seed);
 
  private:
   unsigned n_ = 0;
@@ -57,5 +63,9 @@ class Graph {
   uint64_t edges_ = 0;
   std::vector<Word> bits_;
 };
+
+// The first n of a Fisher-Yates shuffle of 0 .. total - 1 on the stream StreamSeed(seed, kSampleStream, 0), or all of
+// them in order when n = 0 or n >= total (reference src/cc_graph.cpp:373-381).
+std::vector<unsigned> SampleWithoutReplacement(unsigned total, unsigned n, uint64_t seed);
 
 }  // namespace cc

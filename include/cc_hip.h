@@ -205,6 +205,18 @@ uint64_t cc_graph_sbm(uint32_t n, int32_t blocks, double p_in, double p_out, uin
                       int32_t threads);
 uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_t seed, uint64_t* bits,
                            int32_t threads);
+/* Tag-similarity graph, the reference's Graph::LoadTags (include/cc_graph.hpp:51, src/cc_graph.cpp:311-371): n
+ * sampled objects of a {"key": ["tag", ...]} JSON file (n = 0: all), joined when the similarity ("jaccard",
+ * "cosine", "dice" or "overlap") of their tag sets is >= threshold. Sets *n_out and *edges_out; fills bits
+ * (*n_out * cc_words64(*n_out) words) if capacity_words is enough, else returns CC_HIP_ENOMEM so the caller can size
+ * the buffer and call again. CC_HIP_EINVAL for an unreadable or malformed file or an unknown similarity. */
+int cc_graph_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
+                       uint64_t* bits, uint64_t capacity_words, uint32_t* n_out, uint64_t* edges_out, char* err,
+                       size_t err_len);
+/* The reference's SampleWithoutReplacement (include/cc_graph.hpp:62): out receives the first min(n, total) entries
+ * of a partial Fisher-Yates shuffle of 0 .. total - 1 on StreamSeed(seed, 0x7A65, 0), or 0 .. total - 1 when n = 0
+ * or n >= total. */
+void cc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out);
 int64_t cc_objective_direct(const uint64_t* bits, uint32_t n, const int32_t* labels);
 int64_t cc_objective_bitwise(const uint64_t* bits, uint32_t n, const int32_t* labels);
 

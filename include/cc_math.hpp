@@ -64,6 +64,8 @@ constexpr uint64_t kSelectStream = 0x5E1EC7;
 constexpr uint64_t kPerturbStream = 0xBEEF;
 constexpr uint64_t kPerturbFirstStep = 0x5000000;
 constexpr uint64_t kErdosRenyiStream = 0xE4D05F;
+// Object sample of the tag-similarity graph (reference src/cc_graph.cpp:22).
+constexpr uint64_t kSampleStream = 0x7A65;
 // Streams of the generators this project adds.
 constexpr uint64_t kSbmStream = 0x5B3D10C;
 constexpr uint64_t kChungLuStream = 0xC1A9E5;

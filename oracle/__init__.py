@@ -109,6 +109,9 @@ class Oracle:
         lib.orc_rand_float.restype = ctypes.c_float
         lib.orc_rand_below.argtypes = [_u64p, ctypes.c_uint32]
         lib.orc_rand_below.restype = ctypes.c_uint32
+        lib.orc_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                                       ctypes.POINTER(ctypes.c_uint32)]
+        lib.orc_sample_without_replacement.restype = ctypes.c_uint32
         for name in ("orc_graph_erdos_renyi",):
             getattr(lib, name).argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, _u64p, ctypes.c_int]
             getattr(lib, name).restype = ctypes.c_uint64
@@ -147,6 +150,11 @@ class Oracle:
 
     def stream_seed(self, seed, stream, step):
         return self.lib.orc_stream_seed(seed, stream, step)
+
+    def sample_without_replacement(self, total, n, seed):
+        out = np.zeros(max(total, 1), dtype=np.uint32)
+        count = self.lib.orc_sample_without_replacement(total, n, seed, _ptr(out, ctypes.POINTER(ctypes.c_uint32)))
+        return out[:count]
 
     def initial_label(self, base, index, k):
         return self.lib.orc_initial_label(base, index, k)
@@ -272,6 +280,12 @@ class Reference:
         lib.ref_tournament.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _i64p, ctypes.c_int32,
                                        ctypes.c_int32]
         lib.ref_tournament.restype = ctypes.c_int32
+        lib.ref_load_tags.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_uint32,
+                                      ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.c_char_p, ctypes.c_size_t]
+        lib.ref_load_tags.restype = ctypes.c_int64
+        lib.ref_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                                       ctypes.POINTER(ctypes.c_uint32)]
 
     def erdos_renyi(self, n, p, seed):
         words = words_per_row(n)
@@ -325,6 +339,23 @@ class Reference:
         order = np.zeros(len(f), dtype=np.int32)
         self.lib.ref_rank_slots(_ptr(f, _i64p), len(f), _ptr(order, _i32p))
         return order
+
+    def load_tags(self, path, similarity, threshold, n, seed):
+        """Graph::LoadTags of the reference; raises RuntimeError with its message."""
+        edges = ctypes.c_uint64(0)
+        err = ctypes.create_string_buffer(512)
+        args = (os.fsencode(path), similarity.encode(), threshold, n, seed)
+        size = self.lib.ref_load_tags(*args, None, 0, ctypes.byref(edges), err, 512)
+        if size < 0:
+            raise RuntimeError(err.value.decode())
+        bits = np.zeros((size, words_per_row(size)), dtype=np.uint64)
+        self.lib.ref_load_tags(*args, _ptr(bits, _u64p), bits.size, ctypes.byref(edges), err, 512)
+        return Graph(bits, size, edges.value)
+
+    def sample_without_replacement(self, total, n, seed):
+        out = np.zeros(max(total, 1), dtype=np.uint32)
+        self.lib.ref_sample_without_replacement(total, n, seed, _ptr(out, ctypes.POINTER(ctypes.c_uint32)))
+        return out[: (total if n == 0 or n >= total else n)]
 
 
 def reference_available():

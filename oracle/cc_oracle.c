@@ -65,6 +65,25 @@ uint64_t orc_stream_seed(uint64_t seed, uint64_t stream, uint64_t step) {
 #define ORC_SBM_STREAM 0x5B3D10Cull
 #define ORC_CHUNG_LU_STREAM 0xC1A9E5ull
 
+/* src/cc_graph.cpp:22 */
+#define ORC_SAMPLE_STREAM 0x7A65ull
+
+/* src/cc_graph.cpp:373-381: the first n of a Fisher-Yates shuffle of 0 .. total - 1, swapping position i with
+   i + RandBelow(total - i) on the stream StreamSeed(seed, kSampleStream, 0); 0 .. total - 1 in order when n = 0 or
+   n >= total. Returns the number of entries written to out. */
+uint32_t orc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
+  for (uint32_t i = 0; i < total; ++i) out[i] = i;
+  if (n == 0 || n >= total) return total;
+  uint64_t state = orc_stream_seed(seed, ORC_SAMPLE_STREAM, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t j = i + orc_rand_below(&state, total - i);
+    const uint32_t t = out[i];
+    out[i] = out[j];
+    out[j] = t;
+  }
+  return n;
+}
+
 /* include/cc_math.hpp:123-126 */
 int orc_initial_label(uint64_t base, uint64_t index, int k) {
   uint64_t rng = orc_stream_seed(base, index, 0);

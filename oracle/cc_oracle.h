@@ -35,6 +35,7 @@ uint32_t orc_next_u32(uint64_t* state);
 uint32_t orc_rand_below(uint64_t* state, uint32_t bound);
 float orc_rand_float(uint64_t* state);
 uint64_t orc_stream_seed(uint64_t seed, uint64_t stream, uint64_t step);
+uint32_t orc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out);
 int orc_initial_label(uint64_t base, uint64_t index, int k);
 int orc_tournament(uint64_t base, int slot, int iteration, const int64_t* f, int population, int tournament);
 int orc_perturbed_label(uint64_t base, uint64_t index, int iteration, int label, int k, float probability);

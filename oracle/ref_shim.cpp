@@ -147,6 +147,28 @@ int64_t ref_local_search(const uint64_t* bits, uint32_t n, uint32_t words, int32
   return state.f;
 }
 
+// The reference's tag-similarity graph (Graph::LoadTags) as a 64-bit-word bitset: returns -1 with a message, or the
+// vertex count; bits are written only if capacity_words holds them.
+int64_t ref_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
+                      uint64_t* bits, uint64_t capacity_words, uint64_t* edges_out, char* err, size_t err_len) {
+  try {
+    const cc::Graph graph = cc::Graph::LoadTags(path, similarity, threshold, n, seed);
+    const uint32_t size = graph.Size();
+    const uint32_t words = (size + 63) / 64 > 0 ? (size + 63) / 64 : 1;
+    *edges_out = graph.EdgeCount();
+    if ((uint64_t)size * words <= capacity_words) ToBits(graph, bits, words);
+    return size;
+  } catch (const std::exception& e) {
+    SetError(err, err_len, e.what());
+    return -1;
+  }
+}
+
+void ref_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
+  const std::vector<unsigned> order = cc::SampleWithoutReplacement(total, n, seed);
+  for (size_t i = 0; i < order.size(); ++i) out[i] = order[i];
+}
+
 int64_t ref_objective_direct(const uint64_t* bits, uint32_t n, uint32_t words, const int32_t* labels) {
   const cc::Graph graph = FromBits(bits, n, words);
   return cc::ObjectiveDirect(graph, std::vector<int>(labels, labels + n));

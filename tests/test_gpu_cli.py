@@ -80,3 +80,31 @@ def test_invalid_input_is_an_error(tmp_path):
     assert proc.returncode == 1 and "error:" in proc.stdout
     proc = run(["--graph", "missing.txt"], tmp_path, expect_ok=False)
     assert proc.returncode == 1 and "error:" in proc.stdout
+
+
+def test_tags_graph_through_cli(cga, orc, cpu_threads, tmp_path):
+    """--tags builds the tag-similarity graph of the reference's CLI (include/cc_cli.hpp:138-143, 239-241): --n
+    sampled objects by --graph-seed, --similarity and --threshold; the graph saved by the CLI is the library's
+    load_tags graph and the solve on it matches the oracle."""
+    import random
+
+    import oracle
+
+    rng = random.Random(3)
+    vocabulary = [f"t{i}" for i in range(15)]
+    objects = {str(i): [rng.choice(vocabulary) for _ in range(rng.randint(0, 4))] for i in range(260)}
+    (tmp_path / "tags.json").write_text(json.dumps(objects))
+    g = cga.load_tags(tmp_path / "tags.json", "dice", 0.6, 200, 11)
+    want = orc.solve_pbils(oracle.Graph(g.bits, g.n, g.edges), threads=cpu_threads, k=3, population=16,
+                           iterations=5, early_stop=5, seed=7, gww=0.1)
+    out = run(["--tags", "tags.json", "--similarity", "dice", "--threshold", 0.6, "--n", 200, "--graph-seed", 11,
+               *PBILS, "--save-edges", "t.edges"], tmp_path).stdout
+    assert objective_line(out) == want["objective"]
+    saved = np.zeros((g.n, g.n), dtype=bool)
+    for line in (tmp_path / "t.edges").read_text().splitlines()[1:]:
+        i, j = map(int, line.split())
+        saved[i, j] = saved[j, i] = True
+    dense = np.unpackbits(g.bits.view(np.uint8), axis=1, bitorder="little")[:, : g.n].astype(bool)
+    assert (saved == dense).all()
+    proc = run(["--tags", "tags.json", "--similarity", "euclid", *PBILS], tmp_path, expect_ok=False)
+    assert proc.returncode == 1 and "error:" in proc.stdout
