@@ -324,12 +324,55 @@ def bench_neighborhood(cga, args):
     print(json.dumps(out))
 
 
+def bench_tags(cga, args):
+    """The tag-similarity graph builder (Graph::LoadTags) on a synthetic tag file shaped like a photo-tag dataset:
+    objects with 0-8 tags drawn Zipf-like from a vocabulary, jaccard >= 0.5 over every object, beside the reference's
+    own LoadTags (oracle/_ref) on the same file; both graphs must be identical. Host code, all cores. One JSON
+    line."""
+    import random
+    import tempfile
+
+    import numpy as np
+
+    objects, vocabulary = args.tags_objects, 2000
+    rng = random.Random(17)
+    weights = [1.0 / (r + 1) for r in range(vocabulary)]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "tags.json")
+        with open(path, "w") as out:
+            out.write("{\n")
+            for i in range(objects):
+                tags = rng.choices(range(vocabulary), weights, k=rng.randint(0, 8))
+                out.write(f'  "{i}": [' + ", ".join(f'"tag{t}"' for t in tags) + "]" +
+                          (",\n" if i + 1 < objects else "\n"))
+            out.write("}\n")
+        t0 = time.perf_counter()
+        g = cga.load_tags(path, "jaccard", 0.5, 0, 1)
+        ours = time.perf_counter() - t0
+        out = {"metric": "tag-similarity graph build (Graph::LoadTags)", "value": round(ours, 3), "unit": "s",
+               "higher_is_better": False, "data": "synthetic",
+               "config": {"workload": f"{objects} objects, 0-8 tags each from {vocabulary} (weights 1/rank), "
+                                      "jaccard >= 0.5, all objects", "n": g.n, "edges": g.edges},
+               "threads": os.cpu_count()}
+        ref = None if args.no_cpu else reference_solver()
+        if ref is not None:
+            t0 = time.perf_counter()
+            want = ref.load_tags(path, "jaccard", 0.5, 0, 1)
+            theirs = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(theirs, 3), "unit": "s", "cores": os.cpu_count(),
+                                   "kind": "reference", "sample": "the same file through the reference's LoadTags",
+                                   "identical": bool(want.n == g.n and want.edges == g.edges and
+                                                     np.array_equal(want.bits, g.bits))}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "tags"])
+    ap.add_argument("--tags-objects", type=int, default=60000, help="objects of the --config tags file")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
     ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
@@ -353,6 +396,9 @@ def main():
 
     import cuda_graph_clustering_amd as cga
 
+    if args.config == "tags":
+        bench_tags(cga, args)
+        return
     if args.config == "c3":
         bench_neighborhood(cga, args)
         return

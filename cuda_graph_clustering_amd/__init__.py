@@ -141,6 +141,7 @@ ABI_SYMBOLS = [
     "cc_objective_direct",
     "cc_objective_bitwise",
     "cc_graph_load_tags",
+    "cc_host_free",
     "cc_sample_without_replacement",
     "cc_hip_device_info",
     "cc_hip_autotune",
@@ -196,9 +197,10 @@ def _load():
         getattr(lib, name).argtypes = [_u64p, ctypes.c_uint32, _i32p]
         getattr(lib, name).restype = ctypes.c_int64
     lib.cc_graph_load_tags.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_uint32,
-                                       ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.c_uint64, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_uint32),
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_graph_load_tags.restype = ctypes.c_int
+    lib.cc_host_free.argtypes = [ctypes.c_void_p]
     lib.cc_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                                   ctypes.POINTER(ctypes.c_uint32)]
     lib.cc_hip_device_info.argtypes = [ctypes.POINTER(DeviceInfo), ctypes.c_char_p, ctypes.c_size_t]
@@ -261,14 +263,15 @@ def load_tags(path, similarity="jaccard", threshold=0.5, n=0, seed=1):
     seed (0: all), an edge when the similarity of two tag sets is >= threshold."""
     size = ctypes.c_uint32(0)
     edges = ctypes.c_uint64(0)
+    buf = _u64p()
     err = ctypes.create_string_buffer(512)
-    args = (os.fsencode(path), similarity.encode(), float(threshold), int(n), int(seed))
-    rc = lib.cc_graph_load_tags(*args, None, 0, ctypes.byref(size), ctypes.byref(edges), err, 512)
-    if rc not in (0, ENOMEM):
-        _check(rc, err)
-    bits = np.zeros((size.value, words64(size.value)), dtype=np.uint64)
-    _check(lib.cc_graph_load_tags(*args, bits.ctypes.data_as(_u64p), bits.size, ctypes.byref(size),
-                                  ctypes.byref(edges), err, 512), err)
+    _check(lib.cc_graph_load_tags(os.fsencode(path), similarity.encode(), float(threshold), int(n), int(seed),
+                                  ctypes.byref(buf), ctypes.byref(size), ctypes.byref(edges), err, 512), err)
+    try:
+        count = size.value * words64(size.value)
+        bits = np.ctypeslib.as_array(buf, shape=(count,)).copy() if count else np.zeros(0, dtype=np.uint64)
+    finally:
+        lib.cc_host_free(buf)
     return Graph(bits, size.value, edges.value)
 
 

@@ -6,6 +6,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -661,24 +662,25 @@ uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_
 }
 
 int cc_graph_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
-                       uint64_t* bits, uint64_t capacity_words, uint32_t* n_out, uint64_t* edges_out, char* err,
-                       size_t err_len) {
+                       uint64_t** bits_out, uint32_t* n_out, uint64_t* edges_out, char* err, size_t err_len) {
+  *bits_out = nullptr;
   try {
     const cc::Graph g = cc::Graph::LoadTags(path, similarity, threshold, n, seed);
+    const std::vector<cc::Word>& bits = g.Bits();
+    uint64_t* out = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, bits.size()) * sizeof(uint64_t)));
+    if (!out) throw std::runtime_error("out of host memory for the bitset");
+    std::copy(bits.begin(), bits.end(), out);
+    *bits_out = out;
     *n_out = g.Size();
     *edges_out = g.EdgeCount();
-    const uint64_t need = static_cast<uint64_t>(g.Size()) * g.WordsPerRow();
-    if (capacity_words < need) {
-      if (err && err_len) std::snprintf(err, err_len, "bits needs %llu words", static_cast<unsigned long long>(need));
-      return CC_HIP_ENOMEM;
-    }
-    std::copy(g.Bits().begin(), g.Bits().end(), bits);
     return CC_HIP_OK;
   } catch (const std::exception& e) {
     if (err && err_len) std::snprintf(err, err_len, "%s", e.what());
     return CC_HIP_EINVAL;
   }
 }
+
+void cc_host_free(void* p) { std::free(p); }
 
 void cc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
   const std::vector<unsigned> order = cc::SampleWithoutReplacement(total, n, seed);

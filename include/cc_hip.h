@@ -207,12 +207,12 @@ uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_
                            int32_t threads);
 /* Tag-similarity graph, the reference's Graph::LoadTags (include/cc_graph.hpp:51, src/cc_graph.cpp:311-371): n
  * sampled objects of a {"key": ["tag", ...]} JSON file (n = 0: all), joined when the similarity ("jaccard",
- * "cosine", "dice" or "overlap") of their tag sets is >= threshold. Sets *n_out and *edges_out; fills bits
- * (*n_out * cc_words64(*n_out) words) if capacity_words is enough, else returns CC_HIP_ENOMEM so the caller can size
- * the buffer and call again. CC_HIP_EINVAL for an unreadable or malformed file or an unknown similarity. */
+ * "cosine", "dice" or "overlap") of their tag sets is >= threshold. On success *bits_out is a host buffer of
+ * *n_out * cc_words64(*n_out) words that the caller releases with cc_host_free. CC_HIP_EINVAL for an unreadable or
+ * malformed file or an unknown similarity. */
 int cc_graph_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
-                       uint64_t* bits, uint64_t capacity_words, uint32_t* n_out, uint64_t* edges_out, char* err,
-                       size_t err_len);
+                       uint64_t** bits_out, uint32_t* n_out, uint64_t* edges_out, char* err, size_t err_len);
+void cc_host_free(void* p);
 /* The reference's SampleWithoutReplacement (include/cc_graph.hpp:62): out receives the first min(n, total) entries
  * of a partial Fisher-Yates shuffle of 0 .. total - 1 on StreamSeed(seed, 0x7A65, 0), or 0 .. total - 1 when n = 0
  * or n >= total. */
