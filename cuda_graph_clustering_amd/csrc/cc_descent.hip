@@ -880,8 +880,9 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   int sz[3] = {sizes_in[0], sizes_in[1], sizes_in[2]};
   const int64_t f_start = Uniform(r.pop.f[src]);
 
-  // The six pair keys of a chunk from its lanes' words, into the table.
-  auto summarize = [&](int chunk, uint64_t w, bool valid) {
+  // The pair keys of a chunk from its lanes' words, into the table: the pairs whose source label is in the bit
+  // set `labels` (uniform); the entries of the other pairs stay as they are.
+  auto summarize = [&](int chunk, uint64_t w, bool valid, int labels) {
     const int a = valid ? static_cast<int>(w >> kK3LabelShift) : 3;
     const int g[3] = {static_cast<int>(w & kK3Mask), static_cast<int>((w >> kK3Bits) & kK3Mask),
                       static_cast<int>((w >> (2 * kK3Bits)) & kK3Mask)};
@@ -889,6 +890,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
     const uint32_t tie = static_cast<uint32_t>(63 - lane);
 #pragma unroll
     for (int p = 0; p < kPairs3; ++p) {
+      if (!((labels >> (p >> 1)) & 1)) continue;  // uniform
       const uint32_t key =
           a == (p >> 1) ? (static_cast<uint32_t>(g[PairTarget3(p)] - ga + (1 << kK3Bits)) << 6) | tie : 0u;
       const uint32_t best = WaveMax32(key);
@@ -904,7 +906,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
           static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) | static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
       st[u] = w;
     }
-    summarize(chunk, w, u < n);
+    summarize(chunk, w, u < n, 7);
   }
 
   // The thread's candidate: over its chunks of the table, each pair's best entry, as moves. Across chunks only the
@@ -969,14 +971,22 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
       const uint64_t word = row[chunk];  // wave-uniform
       const int u = chunk * kWave + lane;
       uint64_t w = 0;
+      bool neighbour = false;
+      int a = 3;
       if (u < n) {
         w = st[u];
-        const bool neighbour = (word >> lane) & 1ull;
+        a = static_cast<int>(w >> kK3LabelShift);
+        neighbour = (word >> lane) & 1ull;
         if (neighbour) w += inc;
         if (u == v) w = (w & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
         if (neighbour || u == v) st[u] = w;
       }
-      summarize(chunk, w, u < n);
+      // Only the pairs of a changed lane's label can change: a neighbour keeps its label (its G moved), the mover
+      // leaves `from` and joins `to`.
+      int labels = chunk == (v >> 6) ? (1 << from) | (1 << to) : 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) labels |= __ballot(neighbour && a == c) ? 1 << c : 0;
+      summarize(chunk, w, u < n, labels);
     }
     // the other list's counter was last read before this move's first barrier
     if (tid == 0) dirty_count[parity ^ 1] = 0;
