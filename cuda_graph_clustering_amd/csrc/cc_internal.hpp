@@ -101,13 +101,12 @@ void LaunchResetQueues(int32_t* queues, int count, hipStream_t stream);
 void LaunchGatherCtl(const RunDev* runs, int nruns, Ctl* out, hipStream_t stream);
 void LaunchDegrees(const RunDev* run, int n, hipStream_t stream);
 void LaunchInitLabels(const RunDev* runs, const ItemList& items, int max_n, hipStream_t stream);
-// G, sizes and f of `pop` for the items; `pairs` lists (run, first slot) of slot pairs of the same run, grouped by
-// the clusters per block (kc = k for k in [2, 8], 8 above) and G width of their run.
+// G, sizes and f of `pop` for the items; `tiles` lists (run, first column) of the kRebuildCols-column tiles of every
+// run with this G width, column q = slot * k + cluster, in order of run and column.
+constexpr int kRebuildCols = 64;
 struct RebuildGroup {
-  ItemList pairs;
-  int kc;
+  ItemList tiles;
   int wide;
-  int max_k;
 };
 void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream);
 void LaunchRebuildG(const RunDev* runs, const RebuildGroup& group, int max_n, hipStream_t stream);

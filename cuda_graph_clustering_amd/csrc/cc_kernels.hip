@@ -87,79 +87,87 @@ __global__ void __launch_bounds__(kBlock) KernelMasks(const RunDev* runs, const 
   if (threadIdx.x == 0) r.intra[s] = 0;
 }
 
-// Rebuild, part 2: G[s][c][v] = sum_w popc(row_v[w] & mask_{s,c}[w]) for a tile of kRebuildRows vertices x two
-// slots of one run x kClusters clusters. The masks of the tile are staged through LDS in chunks of kWordTile words;
-// every thread owns one row. The diagonal G[s][label v][v] is summed into intra[s] for the objective.
-constexpr int kRebuildRows = 256;
-constexpr int kRebuildSlots = 2;
-constexpr int kWordTile = 256;
+// Rebuild, part 2: G[s][c][v] = sum_w popc(row_v[w] & mask_{s,c}[w]), an AND-popcount product of the adjacency
+// (n rows of W words) with the cluster masks of the columns q = s k + c, which KernelMasks lays out as [q][w], so
+// that G[s][c] is row q of G. A block computes kRebuildRows vertices x kRebuildCols columns of one run (any k),
+// staging kRebuildWords-word slices of both operands through LDS; thread (tx, ty) owns vertices v0 + tx + 16 i and
+// columns q0 + ty + 16 j, i, j < 4, so the adjacency is read once per 64 columns instead of once per slot pair.
+// The diagonal G[s][label v][v] is summed per slot into intra[s] for the objective.
+constexpr int kRebuildRows = 64;
+constexpr int kRebuildWords = 32;
+constexpr int kRebuildThreads = 256;
+static_assert(kRebuildCols == 64 && kRebuildThreads == 16 * 16, "4 x 4 outputs per thread");
 
-template <typename GT, int kClusters>
-__global__ void __launch_bounds__(kRebuildRows) KernelRebuildG(const RunDev* runs, const Item* pairs) {
-  const Item pair = pairs[blockIdx.y];
-  const RunDev& r = runs[pair.run];
+template <typename GT>
+__global__ void __launch_bounds__(kRebuildThreads) KernelRebuildG(const RunDev* runs, const Item* tiles) {
+  const Item tile = tiles[blockIdx.y];
+  const RunDev& r = runs[tile.run];
   if (r.ctl->stopped) return;
-  const int n = r.n, k = r.k, words = r.words, population = r.population;
-  const int c0 = blockIdx.z * kClusters;
-  if (c0 >= k || static_cast<int>(blockIdx.x) * kRebuildRows >= n) return;  // uniform per block
-  const int v = blockIdx.x * kRebuildRows + threadIdx.x;
-  const int s0 = pair.slot;
-  __shared__ uint64_t tile[kRebuildSlots][kClusters][kWordTile];
-  __shared__ int64_t block_intra[kRebuildSlots];
-  if (threadIdx.x < kRebuildSlots) block_intra[threadIdx.x] = 0;
+  const int n = r.n, k = r.k, words = r.words;
+  const int v0 = blockIdx.x * kRebuildRows;
+  const int q0 = tile.slot;                       // first column of the tile
+  const int columns = r.population * k - q0;      // columns of the run from q0 on
+  if (v0 >= n || columns <= 0) return;            // uniform per block
+  __shared__ uint64_t a_lds[kRebuildWords][kRebuildRows + 1];
+  __shared__ uint64_t m_lds[kRebuildWords][kRebuildCols + 1];
+  __shared__ unsigned long long tile_intra[kRebuildCols];  // per slot of the tile (k >= 1: at most 64 slots)
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  if (tid < kRebuildCols) tile_intra[tid] = 0;
 
-  int acc[kRebuildSlots][kClusters];
+  int acc[4][4];
 #pragma unroll
-  for (int i = 0; i < kRebuildSlots; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int c = 0; c < kClusters; ++c) acc[i][c] = 0;
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0;
 
-  const uint64_t* row = r.bits + static_cast<size_t>(v < n ? v : 0) * words;
-  for (int w0 = 0; w0 < words; w0 += kWordTile) {
-    const int span = words - w0 < kWordTile ? words - w0 : kWordTile;
+  for (int w0 = 0; w0 < words; w0 += kRebuildWords) {
+    const int span = words - w0 < kRebuildWords ? words - w0 : kRebuildWords;
     __syncthreads();
-    for (int i = threadIdx.x; i < kRebuildSlots * kClusters * kWordTile; i += kRebuildRows) {
-      const int w = i % kWordTile;
-      const int c = (i / kWordTile) % kClusters;
-      const int si = i / (kWordTile * kClusters);
-      const int s = s0 + si;
-      uint64_t m = 0;
-      if (w < span && s < population && c0 + c < k) {
-        m = r.masks[(static_cast<size_t>(s) * k + c0 + c) * words + w0 + w];
-      }
-      tile[si][c][w] = m;
+    // consecutive threads read consecutive words of one row (or column); zero outside the run and the row
+    for (int e = tid; e < kRebuildRows * kRebuildWords; e += kRebuildThreads) {
+      const int row = e / kRebuildWords, w = e % kRebuildWords;
+      const int v = v0 + row;
+      a_lds[w][row] = (v < n && w < span) ? r.bits[static_cast<size_t>(v) * words + w0 + w] : 0ull;
+      m_lds[w][row] = (row < columns && w < span) ? r.masks[static_cast<size_t>(q0 + row) * words + w0 + w] : 0ull;
     }
     __syncthreads();
-    if (v < n) {
-      for (int w = 0; w < span; ++w) {
-        const uint64_t x = row[w0 + w];
+    for (int w = 0; w < span; ++w) {
+      uint64_t a[4], m[4];
 #pragma unroll
-        for (int i = 0; i < kRebuildSlots; ++i)
+      for (int i = 0; i < 4; ++i) a[i] = a_lds[w][tx + 16 * i];
 #pragma unroll
-          for (int c = 0; c < kClusters; ++c) acc[i][c] += __popcll(x & tile[i][c][w]);
-      }
+      for (int j = 0; j < 4; ++j) m[j] = m_lds[w][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += __popcll(a[i] & m[j]);
     }
   }
-  if (v < n) {
+
+  const int s_first = q0 / k;
 #pragma unroll
-    for (int i = 0; i < kRebuildSlots; ++i) {
-      const int s = s0 + i;
-      if (s >= population) continue;
-      GT* g = static_cast<GT*>(r.pop.g) + static_cast<size_t>(s) * k * n;
-      const int label = r.pop.labels[static_cast<size_t>(s) * n + v];
-      int diag = 0;
+  for (int j = 0; j < 4; ++j) {
+    const int qt = ty + 16 * j;
+    if (qt >= columns) continue;
+    const int q = q0 + qt;
+    const int s = q / k, c = q - s * k;
+    GT* g = static_cast<GT*>(r.pop.g) + static_cast<size_t>(q) * n;
+    const uint8_t* labels = r.pop.labels + static_cast<size_t>(s) * n;
+    int diag = 0;
 #pragma unroll
-      for (int c = 0; c < kClusters; ++c) {
-        if (c0 + c < k) g[static_cast<size_t>(c0 + c) * n + v] = static_cast<GT>(acc[i][c]);
-        if (label == c0 + c) diag = acc[i][c];
+    for (int i = 0; i < 4; ++i) {
+      const int v = v0 + tx + 16 * i;
+      if (v < n) {
+        g[v] = static_cast<GT>(acc[i][j]);
+        if (labels[v] == c) diag += acc[i][j];
       }
-      if (diag) atomicAdd(reinterpret_cast<unsigned long long*>(&block_intra[i]), static_cast<unsigned long long>(diag));
     }
+    if (diag) atomicAdd(&tile_intra[s - s_first], static_cast<unsigned long long>(diag));
   }
   __syncthreads();
-  if (threadIdx.x < kRebuildSlots && s0 + static_cast<int>(threadIdx.x) < population && block_intra[threadIdx.x]) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&r.intra[s0 + threadIdx.x]),
-              static_cast<unsigned long long>(block_intra[threadIdx.x]));
+  if (tid < kRebuildCols && tile_intra[tid]) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&r.intra[s_first + tid]), tile_intra[tid]);
   }
 }
 
@@ -373,25 +381,13 @@ void LaunchMasks(const RunDev* runs, const ItemList& items, hipStream_t stream) 
 
 template <typename GT>
 static void LaunchRebuildGT(const RunDev* runs, const RebuildGroup& g, int max_n, hipStream_t stream) {
-  ForChunksY(g.pairs.items, g.pairs.count, [&](const Item* chunk, int count) {
-    const dim3 grid(Blocks(max_n, kRebuildRows), count, 1);
-    switch (g.kc) {  // k <= 8: exactly k clusters per block; k > 8: chunks of 8 along grid z
-      case 2: KernelRebuildG<GT, 2><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      case 3: KernelRebuildG<GT, 3><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      case 4: KernelRebuildG<GT, 4><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      case 5: KernelRebuildG<GT, 5><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      case 6: KernelRebuildG<GT, 6><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      case 7: KernelRebuildG<GT, 7><<<grid, kRebuildRows, 0, stream>>>(runs, chunk); break;
-      default: {
-        const dim3 grid8(grid.x, grid.y, Blocks(g.max_k, 8));
-        KernelRebuildG<GT, 8><<<grid8, kRebuildRows, 0, stream>>>(runs, chunk);
-      }
-    }
+  ForChunksY(g.tiles.items, g.tiles.count, [&](const Item* chunk, int count) {
+    KernelRebuildG<GT><<<dim3(Blocks(max_n, kRebuildRows), count), kRebuildThreads, 0, stream>>>(runs, chunk);
   });
 }
 
 void LaunchRebuildG(const RunDev* runs, const RebuildGroup& group, int max_n, hipStream_t stream) {
-  if (group.pairs.count == 0) return;
+  if (group.tiles.count == 0) return;
   if (group.wide) {
     LaunchRebuildGT<int32_t>(runs, group, max_n, stream);
   } else {

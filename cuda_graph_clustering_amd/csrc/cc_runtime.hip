@@ -344,9 +344,6 @@ class Arena {
   size_t used_ = 0;
 };
 
-// Clusters per rebuild block: exactly k up to 8 (no idle popcounts), chunks of 8 above.
-int KcOf(int k) { return k <= 2 ? 2 : (k <= 8 ? k : 8); }
-
 // Expected relative cost of one descent of a run, to order the persistent queues largest first.
 double DescentCost(const RunSpec& s) {
   const double n = s.graph->n;
@@ -452,13 +449,13 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
 
   // item tables (host)
   std::vector<Item> all;
-  std::map<std::pair<int, int>, std::vector<Item>> pairs;  // (wide, kc) -> slot pairs
+  std::map<int, std::vector<Item>> tiles;  // wide -> (run, first column) of the rebuild tiles
   std::vector<std::pair<DescentPlan, std::vector<Item>>> groups;
   for (int i = 0; i < nruns; ++i) {
     const RunSpec& s = spec(i);
     for (int slot = 0; slot < s.population; ++slot) all.push_back({i, slot});
-    auto& pl = pairs[{s.plan.wide, KcOf(s.params.k)}];
-    for (int slot = 0; slot < s.population; slot += 2) pl.push_back({i, slot});
+    auto& tl = tiles[s.plan.wide];
+    for (int q0 = 0; q0 < s.population * s.params.k; q0 += kRebuildCols) tl.push_back({i, q0});
     size_t g = 0;
     while (g < groups.size() && !SameLaunch(groups[g].first, s.plan)) ++g;
     if (g == groups.size()) groups.push_back({s.plan, {}});
@@ -471,13 +468,13 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
   }
 
   // one allocation for the tables
-  std::vector<Item*> pair_tables(pairs.size()), group_tables(groups.size());
+  std::vector<Item*> tile_tables(tiles.size()), group_tables(groups.size());
   Item* all_table = nullptr;
   auto layout = [&](Arena& a) {
     a.Take(runs_, nruns);
     a.Take(all_table, all.size());
     size_t i = 0;
-    for (auto& kv : pairs) a.Take(pair_tables[i++], kv.second.size());
+    for (auto& kv : tiles) a.Take(tile_tables[i++], kv.second.size());
     for (size_t g = 0; g < groups.size(); ++g) a.Take(group_tables[g], groups[g].second.size());
     a.Take(queues_, std::max<size_t>(groups.size(), 1));
     a.Take(ctl_table_, nruns);
@@ -491,17 +488,15 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
   CC_HIP_CHECK(hipMemcpyAsync(runs_, host_runs_.data(), sizeof(RunDev) * nruns, hipMemcpyHostToDevice, stream_));
   all_ = ItemList{all_table, static_cast<int32_t>(all.size()), nullptr};
   CC_HIP_CHECK(hipMemcpyAsync(all_table, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice, stream_));
-  size_t pi = 0;
-  for (auto& [key, list] : pairs) {
+  size_t ti = 0;
+  for (auto& [wide, list] : tiles) {
     RebuildGroup g{};
-    g.wide = key.first;
-    g.kc = key.second;
-    g.pairs = ItemList{pair_tables[pi], static_cast<int32_t>(list.size()), nullptr};
-    for (const Item& it : list) g.max_k = std::max(g.max_k, spec(it.run).params.k);
-    CC_HIP_CHECK(hipMemcpyAsync(pair_tables[pi], list.data(), sizeof(Item) * list.size(), hipMemcpyHostToDevice,
+    g.wide = wide;
+    g.tiles = ItemList{tile_tables[ti], static_cast<int32_t>(list.size()), nullptr};
+    CC_HIP_CHECK(hipMemcpyAsync(tile_tables[ti], list.data(), sizeof(Item) * list.size(), hipMemcpyHostToDevice,
                                 stream_));
     rebuild_.push_back(g);
-    ++pi;
+    ++ti;
   }
   int cus = 0;
   CC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, spec(0).graph->device));
@@ -814,7 +809,7 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
     } else {
       for (const RebuildGroup& rg : batch.RebuildGroups()) {
         RebuildGroup prefix = rg;
-        prefix.pairs = batch.Prefix(rg.pairs, (count + 1) / 2);
+        prefix.tiles = batch.Prefix(rg.tiles, (count * params.k + kRebuildCols - 1) / kRebuildCols);
         LaunchRebuildG(batch.Runs(), prefix, n, s);
       }
     }
