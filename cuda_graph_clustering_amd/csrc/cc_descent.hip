@@ -864,7 +864,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   __shared__ int32_t dirty_count[2];
   __shared__ int32_t src_sh;
   extern __shared__ __align__(16) uint8_t dyn[];
-  uint32_t* table = reinterpret_cast<uint32_t*>(dyn);               // [kPairs3][chunks]
+  uint32_t* table = reinterpret_cast<uint32_t*>(dyn);               // [chunks][kPairs3]
   int32_t* dirty = reinterpret_cast<int32_t*>(table + kPairs3 * chunks);  // [2][chunks]
   if (tid == 0) {
     src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
@@ -877,7 +877,11 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
   const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * n);
-  int sz[3] = {sizes_in[0], sizes_in[1], sizes_in[2]};
+  // the three cluster sizes (< 2^21) as 21-bit fields of one uniform word: a move updates them with scalar adds
+  uint64_t szp = static_cast<uint64_t>(sizes_in[0]) | static_cast<uint64_t>(sizes_in[1]) << 21 |
+                 static_cast<uint64_t>(sizes_in[2]) << 42;
+  szp = static_cast<uint64_t>(Uniform(static_cast<int64_t>(szp)));
+  auto size_of = [&](int c) { return static_cast<int>((szp >> (21 * c)) & 0x1FFFFFu); };  // c known at compile time
   const int64_t f_start = Uniform(r.pop.f[src]);
 
   // The pair keys of a chunk from its lanes' words, into the table: the pairs whose source label is in the bit
@@ -894,7 +898,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
       const uint32_t key =
           a == (p >> 1) ? (static_cast<uint32_t>(g[PairTarget3(p)] - ga + (1 << kK3Bits)) << 6) | tie : 0u;
       const uint32_t best = WaveMax32(key);
-      if (lane == 0) table[p * chunks + chunk] = best;
+      if (lane == 0) table[chunk * kPairs3 + p] = best;
     }
   };
 
@@ -913,24 +917,36 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   // gain (key >> 6) is compared and the first chunk wins a tie: the low bits order vertices within one chunk only,
   // and every vertex of an earlier chunk is smaller than every vertex of a later one.
   auto candidate = [&]() {
+    uint32_t m[kPairs3];
+    int mc[kPairs3];
+#pragma unroll
+    for (int p = 0; p < kPairs3; ++p) {
+      m[p] = 0;
+      mc[p] = 0;
+    }
+    // one pass over the thread's chunks reads the six entries of each (three 8-byte reads)
+#pragma unroll 2
+    for (int c = tid; c < chunks; c += T) {
+      const uint2* entry = reinterpret_cast<const uint2*>(table + c * kPairs3);
+      const uint2 e01 = entry[0], e23 = entry[1], e45 = entry[2];
+      const uint32_t k[kPairs3] = {e01.x, e01.y, e23.x, e23.y, e45.x, e45.y};
+#pragma unroll
+      for (int p = 0; p < kPairs3; ++p) {
+        if ((k[p] >> 6) > (m[p] >> 6)) {
+          m[p] = k[p];
+          mc[p] = c;
+        }
+      }
+    }
     uint64_t best = kNoMove;
 #pragma unroll
     for (int p = 0; p < kPairs3; ++p) {
-      uint32_t m = 0;
-      int mc = 0;
-      for (int c = tid; c < chunks; c += T) {
-        const uint32_t k = table[p * chunks + c];
-        if ((k >> 6) > (m >> 6)) {
-          m = k;
-          mc = c;
-        }
-      }
-      if (m) {
+      if (m[p]) {
         const int a = p >> 1, to = PairTarget3(p);
-        const int x = static_cast<int>(m >> 6) - (1 << kK3Bits);
-        const int d = (sz[to] - sz[a] + 1) - 2 * x;
+        const int x = static_cast<int>(m[p] >> 6) - (1 << kK3Bits);
+        const int d = (size_of(to) - size_of(a) + 1) - 2 * x;
         if (d < 0) {
-          const int u = mc * kWave + 63 - static_cast<int>(m & 63u);
+          const int u = mc[p] * kWave + 63 - static_cast<int>(m[p] & 63u);
           best = Min(best, PackMove(d, u, a, to));
         }
       }
@@ -955,8 +971,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
     const int v = static_cast<int>(mv >> 12);
     const int from = static_cast<int>((mv >> 6) & 63u);
     const int to = static_cast<int>(mv & 63u);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sz[c] += (c == to) - (c == from);
+    szp += (uint64_t(1) << (21 * to)) - (uint64_t(1) << (21 * from));
     const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
     const uint64_t* row = row_base + static_cast<size_t>(v) * row_words;
     // the chunks to recompute: every non-zero word of v's row, and v's own chunk
@@ -1013,9 +1028,9 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   }
   if (tid == 0) {
     int32_t* sizes = r.next.sizes + static_cast<size_t>(s) * 3;
-    sizes[0] = sz[0];
-    sizes[1] = sz[1];
-    sizes[2] = sz[2];
+    sizes[0] = size_of(0);
+    sizes[1] = size_of(1);
+    sizes[2] = size_of(2);
     r.next.f[s] = objective;
     r.moves[s] = applied;
   }
