@@ -1089,9 +1089,11 @@ __global__ void __launch_bounds__(T) KernelDescentK3Chunk(const RunDev* runs, It
   Persistent(runs, list, [](const RunDev& r, int s) { DescentK3ChunkBody<T>(r, s); });
 }
 
-// The mixed descent: one persistent 1024-thread launch for a batch whose runs use different register kernels. The
-// queue holds the items of all of them, largest runs first, and each item runs the variant its run names
-// (RunDev::variant), so no CU idles while another kernel instance's launch drains its tail.
+// The mixed descent: one persistent launch for a batch whose runs use different register kernels. The queue holds
+// the items of all of them, largest runs first, and each item runs the variant its run names (RunDev::variant), so
+// no CU idles while another kernel instance's launch drains its tail. Two block sizes: 1024 threads (one
+// workgroup per CU) for any register run, and 512 threads held to 128 VGPRs (two workgroups per CU, so one
+// descent's barrier and row-load latency overlaps the other's work) for runs up to 4096 vertices.
 struct MixedDesc {
   int kind;  // 0: k = 2 registers, 1: k in [3, 8] registers
   int k;
@@ -1108,26 +1110,46 @@ constexpr MixedDesc kMixedTable[] = {
 };
 constexpr int kMixedCount = sizeof(kMixedTable) / sizeof(kMixedTable[0]);
 constexpr int kMixedThreads = 1024;
+constexpr MixedDesc kMixedTable512[] = {
+    {0, 2, 1}, {0, 2, 2}, {0, 2, 4}, {0, 2, 8},  //
+    {1, 3, 1}, {1, 3, 2}, {1, 3, 4}, {1, 3, 8},  //
+    {1, 4, 1}, {1, 4, 2}, {1, 4, 4}, {1, 4, 8},  //
+    {1, 5, 1}, {1, 5, 2}, {1, 5, 4}, {1, 5, 8},  //
+    {1, 6, 1}, {1, 6, 2}, {1, 6, 4}, {1, 6, 8},  //
+    {1, 7, 1}, {1, 7, 2}, {1, 7, 4}, {1, 7, 8},  //
+    {1, 8, 1}, {1, 8, 2}, {1, 8, 4}, {1, 8, 8},  //
+};
+constexpr int kMixedCount512 = sizeof(kMixedTable512) / sizeof(kMixedTable512[0]);
+constexpr int kMixedThreads512 = 512;
 
-template <int I>
+template <int T, const MixedDesc* kTable, int I>
 __device__ __forceinline__ void MixedOne(const RunDev& r, int s) {
-  constexpr MixedDesc d = kMixedTable[I];
+  constexpr MixedDesc d = kTable[I];
   if constexpr (d.kind == 0) {
-    DescentK2Body<kMixedThreads, d.vpt, false>(r, s);
+    DescentK2Body<T, d.vpt, false>(r, s);
   } else {
-    DescentRegsBody<kMixedThreads, d.k, d.vpt>(r, s);
+    DescentRegsBody<T, d.k, d.vpt>(r, s);
   }
 }
 
-template <int... I>
+template <int T, const MixedDesc* kTable, int... I>
 __device__ __forceinline__ void MixedSwitch(const RunDev& r, int s, std::integer_sequence<int, I...>) {
   const int v = r.variant;
-  ((v == I ? (MixedOne<I>(r, s), 0) : 0), ...);
+  ((v == I ? (MixedOne<T, kTable, I>(r, s), 0) : 0), ...);
 }
 
 __global__ void __launch_bounds__(kMixedThreads) KernelDescentMixed(const RunDev* runs, ItemList list) {
-  Persistent(runs, list,
-             [](const RunDev& r, int s) { MixedSwitch(r, s, std::make_integer_sequence<int, kMixedCount>{}); });
+  Persistent(runs, list, [](const RunDev& r, int s) {
+    MixedSwitch<kMixedThreads, kMixedTable>(r, s, std::make_integer_sequence<int, kMixedCount>{});
+  });
+}
+
+// Two workgroups per CU: 16 waves, 4 per SIMD, so at most 128 VGPRs each.
+__global__ void __launch_bounds__(kMixedThreads512) __attribute__((amdgpu_waves_per_eu(4)))
+KernelDescentMixed512(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) {
+    MixedSwitch<kMixedThreads512, kMixedTable512>(r, s, std::make_integer_sequence<int, kMixedCount512>{});
+  });
 }
 
 template <typename Kernel>
@@ -1289,7 +1311,10 @@ bool Dispatch(const Go& go) {
       if (p.threads == 1024) return go.Run(KernelDescentK3Chunk<1024>, 1024, p.lds_bytes);
       return false;
     case Descent::kMixed:
-      return !p.wide && p.threads == kMixedThreads && go.Run(KernelDescentMixed, kMixedThreads, 0);
+      if (p.wide) return false;
+      if (p.threads == kMixedThreads) return go.Run(KernelDescentMixed, kMixedThreads, 0);
+      if (p.threads == kMixedThreads512) return go.Run(KernelDescentMixed512, kMixedThreads512, 0);
+      return false;
   }
   return false;
 }
@@ -1318,20 +1343,24 @@ size_t K3ChunkLdsBytes(int n) {
 }
 
 int MixedVariant(int n, int k, DescentPlan* plan) {
-  for (int i = 0; i < kMixedCount; ++i) {
-    const MixedDesc& d = kMixedTable[i];
-    if (d.k == k && d.vpt * kMixedThreads >= n) {
-      plan->descent = d.kind == 0 ? Descent::kK2Registers : Descent::kRegisters;
-      plan->k = k;
-      plan->threads = kMixedThreads;
-      plan->vpt = d.vpt;
-      plan->rows_in_lds = false;
-      plan->lds_bytes = 0;
-      plan->wide = 0;
-      return i;
+  auto pick = [&](const MixedDesc* table, int count, int threads) {
+    for (int i = 0; i < count; ++i) {
+      const MixedDesc& d = table[i];
+      if (d.k == k && d.vpt * threads >= n) {
+        plan->descent = d.kind == 0 ? Descent::kK2Registers : Descent::kRegisters;
+        plan->k = k;
+        plan->threads = threads;
+        plan->vpt = d.vpt;
+        plan->rows_in_lds = false;
+        plan->lds_bytes = 0;
+        plan->wide = 0;
+        return i;
+      }
     }
-  }
-  return -1;
+    return -1;
+  };
+  const int small = pick(kMixedTable512, kMixedCount512, kMixedThreads512);
+  return small >= 0 ? small : pick(kMixedTable, kMixedCount, kMixedThreads);
 }
 
 int RegistersVpt(int n, int k, int* threads) {

@@ -70,7 +70,7 @@ struct RunDev {
   int32_t do_perturb;  // k >= 2 and perturbation > 0 (src/cc_pbils_cpu.cpp:241)
   int32_t select;      // 1: the descent of slot s starts from Tournament(pop); 0: from pop[s] itself
   int32_t wide;        // 32-bit G
-  int32_t variant;     // index of the run's kernel in the mixed descent (register kernels, 1024 threads), or -1
+  int32_t variant;     // index of the run's kernel in the table of its mixed descent (register kernels), or -1
 };
 
 // A slot of a run: the unit of work of the slot-level kernels and of the descent.
@@ -140,7 +140,7 @@ enum class Descent : int {
   kK2Wide = 3,
   kK3Wide = 4,
   kRegisters = 5,  // k in [3, 8], narrow, state in registers
-  kMixed = 6,      // a batch launch over runs of different register kernels (1024 threads), by RunDev::variant
+  kMixed = 6,      // a batch launch over runs of different register kernels (512 or 1024 threads), by RunDev::variant
   kK3Chunk = 7,    // k = 3 wide, packed words plus per-chunk pair summaries in LDS (K3ChunkLdsBytes)
 };
 
@@ -168,8 +168,9 @@ size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 size_t K3ChunkLdsBytes(int n);
 int RegistersVpt(int n, int k, int* threads);
-// The mixed-descent variant that runs a register plan of (k, n) with 1024 threads, or -1 if none; `plan` receives
-// the equivalent 1024-thread plan.
+// The mixed-descent variant that runs a register plan of (k, n), or -1 if none: an index into the 512-thread table
+// when a 512-thread variant holds n (then plan->threads = 512), else into the 1024-thread table; `plan` receives
+// the equivalent register plan.
 int MixedVariant(int n, int k, DescentPlan* plan);
 
 }  // namespace cc::hip

@@ -841,6 +841,18 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   FillPlan(batch.Plan(0), stats);
 }
 
+void UseMixedDescent(RunSpec& spec) {
+  const bool registers = spec.plan.descent == Descent::kK2Registers || spec.plan.descent == Descent::kRegisters;
+  if (!registers || spec.params.ls_kernel != CC_LS_AUTO) return;
+  DescentPlan variant_plan;
+  const int variant = MixedVariant(static_cast<int>(spec.graph->n), spec.params.k, &variant_plan);
+  if (variant < 0) return;
+  spec.variant = variant;
+  spec.plan = DescentPlan{};
+  spec.plan.descent = Descent::kMixed;
+  spec.plan.threads = variant_plan.threads;
+}
+
 // Many independent PBILS runs advanced together: one launch per phase for all runs, descents grouped by kernel
 // instance and claimed from persistent work queues, iterations replayed from one HIP graph.
 void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
@@ -848,18 +860,7 @@ void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
   for (BatchRun& b : runs) {
     CheckParams(b.params);
     RunSpec spec{b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params), b.params.population, true};
-    // Register-kernel runs of a batch share the mixed launch (same trajectory, 1024 threads per workgroup).
-    const bool registers = spec.plan.descent == Descent::kK2Registers || spec.plan.descent == Descent::kRegisters;
-    if (runs.size() > 1 && registers && b.params.ls_kernel == CC_LS_AUTO) {
-      DescentPlan variant_plan;
-      const int variant = MixedVariant(static_cast<int>(b.graph->n), b.params.k, &variant_plan);
-      if (variant >= 0) {
-        spec.variant = variant;
-        spec.plan = DescentPlan{};
-        spec.plan.descent = Descent::kMixed;
-        spec.plan.threads = 1024;
-      }
-    }
+    if (runs.size() > 1) UseMixedDescent(spec);
     specs.push_back(spec);
   }
   Stream stream;

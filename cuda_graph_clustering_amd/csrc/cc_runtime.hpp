@@ -162,6 +162,10 @@ struct RunSpec {
   int variant = -1;  // mixed-descent variant when plan.descent is kMixed
 };
 
+// A register-kernel run (ls_kernel auto) moved to the mixed descent of its block size (512 threads up to 4096
+// vertices, else 1024), so that the runs of a batch or a serving lane share one or two launches; same trajectory.
+void UseMixedDescent(RunSpec& spec);
+
 struct DescentGroup {
   DescentPlan plan;
   ItemList list;
@@ -187,8 +191,8 @@ class RunState {
   size_t bytes_ = 0;
 };
 
-// The launches over a set of runs: the device run table, the item tables of every launch (slots, rebuild slot
-// pairs, descent items grouped by kernel instance and ordered largest run first) and the persistent-queue
+// The launches over a set of runs: the device run table, the item tables of every launch (slots, rebuild
+// tiles, descent items grouped by kernel instance and ordered largest run first) and the persistent-queue
 // counters. An iteration of all runs is one fixed launch sequence. With a pool and a pinned buffer (both kept by the
 // caller), the set allocates and frees nothing through the HIP runtime.
 class LaunchSet {

@@ -211,10 +211,10 @@ class Server {
           Upload(*a.graph, req.bits.data(), req.n, req.edges, s, &pool_);
           req.bits.clear();
           req.bits.shrink_to_fit();
-          a.state = std::make_unique<RunState>(
-              RunSpec{a.graph.get(), req.params, PlanDescent(req.n, req.params.k, req.params), req.params.population,
-                      true},
-              s, &pool_);
+          RunSpec spec{a.graph.get(), req.params, PlanDescent(req.n, req.params.k, req.params),
+                       req.params.population, true};
+          UseMixedDescent(spec);  // the lane's register runs share the mixed launches
+          a.state = std::make_unique<RunState>(spec, s, &pool_);
         } catch (const std::exception& e) {
           Result r;
           r.code = CC_HIP_EDEVICE;

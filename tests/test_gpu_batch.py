@@ -60,3 +60,22 @@ def test_batch_of_one_equals_solve(cga):
         kw = dict(k=5, population=32, iterations=6, early_stop=6, gww=0.1, seed=42)
         (res,), _ = cga.solve_batch([dg], [kw])
         same(res, dg.solve_pbils(**kw))
+
+
+def test_mixed_batch_spans_both_block_sizes(cga):
+    """Register runs up to 4096 vertices take the 512-thread mixed descent (two workgroups per CU), larger ones the
+    1024-thread one; both in one batch, each run equal to its solo solve."""
+    shapes = [(1500, 2), (4096, 5), (4097, 8), (4097, 2), (7000, 3), (2500, 7)]
+    graphs, params = [], []
+    for i, (n, k) in enumerate(shapes):
+        graphs.append(cga.DeviceGraph(cga.sbm(n, 4, 0.3, 0.05, 30 + i)))
+        params.append(dict(k=k, population=12, iterations=4, early_stop=4, gww=0.1, seed=100 + i))
+    try:
+        results, _ = cga.solve_batch(graphs, params)
+        for (n, _), g, kw, res in zip(shapes, graphs, params, results):
+            same(res, g.solve_pbils(**kw))
+            assert res["descent_name"] == "mixed"
+            assert res["threads"] == (512 if n <= 4096 else 1024), n
+    finally:
+        for g in graphs:
+            g.close()
