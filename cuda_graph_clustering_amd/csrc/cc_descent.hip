@@ -1091,9 +1091,10 @@ __global__ void __launch_bounds__(T) KernelDescentK3Chunk(const RunDev* runs, It
 
 // The mixed descent: one persistent launch for a batch whose runs use different register kernels. The queue holds
 // the items of all of them, largest runs first, and each item runs the variant its run names (RunDev::variant), so
-// no CU idles while another kernel instance's launch drains its tail. Two block sizes: 1024 threads (one
-// workgroup per CU) for any register run, and 512 threads held to 128 VGPRs (two workgroups per CU, so one
-// descent's barrier and row-load latency overlaps the other's work) for runs up to 4096 vertices.
+// no CU idles while another kernel instance's launch drains its tail. Three block sizes: 1024 threads (one
+// workgroup per CU) for any register run, and 512 and 256 threads held to 128 VGPRs (two and four workgroups per
+// CU, so one descent's barrier and row-load latency overlaps the others' work) for runs up to 4096 and 2048
+// vertices.
 struct MixedDesc {
   int kind;  // 0: k = 2 registers, 1: k in [3, 8] registers
   int k;
@@ -1121,6 +1122,10 @@ constexpr MixedDesc kMixedTable512[] = {
 };
 constexpr int kMixedCount512 = sizeof(kMixedTable512) / sizeof(kMixedTable512[0]);
 constexpr int kMixedThreads512 = 512;
+// the same variants with 256 threads hold n <= 2048
+constexpr const MixedDesc* kMixedTable256 = kMixedTable512;
+constexpr int kMixedCount256 = kMixedCount512;
+constexpr int kMixedThreads256 = 256;
 
 template <int T, const MixedDesc* kTable, int I>
 __device__ __forceinline__ void MixedOne(const RunDev& r, int s) {
@@ -1144,11 +1149,18 @@ __global__ void __launch_bounds__(kMixedThreads) KernelDescentMixed(const RunDev
   });
 }
 
-// Two workgroups per CU: 16 waves, 4 per SIMD, so at most 128 VGPRs each.
+// Two (four) workgroups per CU: 16 waves, 4 per SIMD, so at most 128 VGPRs each.
 __global__ void __launch_bounds__(kMixedThreads512) __attribute__((amdgpu_waves_per_eu(4)))
 KernelDescentMixed512(const RunDev* runs, ItemList list) {
   Persistent(runs, list, [](const RunDev& r, int s) {
     MixedSwitch<kMixedThreads512, kMixedTable512>(r, s, std::make_integer_sequence<int, kMixedCount512>{});
+  });
+}
+
+__global__ void __launch_bounds__(kMixedThreads256) __attribute__((amdgpu_waves_per_eu(4)))
+KernelDescentMixed256(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) {
+    MixedSwitch<kMixedThreads256, kMixedTable512>(r, s, std::make_integer_sequence<int, kMixedCount256>{});
   });
 }
 
@@ -1314,6 +1326,7 @@ bool Dispatch(const Go& go) {
       if (p.wide) return false;
       if (p.threads == kMixedThreads) return go.Run(KernelDescentMixed, kMixedThreads, 0);
       if (p.threads == kMixedThreads512) return go.Run(KernelDescentMixed512, kMixedThreads512, 0);
+      if (p.threads == kMixedThreads256) return go.Run(KernelDescentMixed256, kMixedThreads256, 0);
       return false;
   }
   return false;
@@ -1359,6 +1372,8 @@ int MixedVariant(int n, int k, DescentPlan* plan) {
     }
     return -1;
   };
+  const int smallest = pick(kMixedTable256, kMixedCount256, kMixedThreads256);
+  if (smallest >= 0) return smallest;
   const int small = pick(kMixedTable512, kMixedCount512, kMixedThreads512);
   return small >= 0 ? small : pick(kMixedTable, kMixedCount, kMixedThreads);
 }

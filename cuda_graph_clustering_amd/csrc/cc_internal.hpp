@@ -140,7 +140,7 @@ enum class Descent : int {
   kK2Wide = 3,
   kK3Wide = 4,
   kRegisters = 5,  // k in [3, 8], narrow, state in registers
-  kMixed = 6,      // a batch launch over runs of different register kernels (512 or 1024 threads), by RunDev::variant
+  kMixed = 6,      // a batch launch over runs of different register kernels (256/512/1024 threads), by RunDev::variant
   kK3Chunk = 7,    // k = 3 wide, packed words plus per-chunk pair summaries in LDS (K3ChunkLdsBytes)
 };
 
@@ -168,8 +168,8 @@ size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 size_t K3ChunkLdsBytes(int n);
 int RegistersVpt(int n, int k, int* threads);
-// The mixed-descent variant that runs a register plan of (k, n), or -1 if none: an index into the 512-thread table
-// when a 512-thread variant holds n (then plan->threads = 512), else into the 1024-thread table; `plan` receives
+// The mixed-descent variant that runs a register plan of (k, n), or -1 if none: an index into the table of the launch
+// with the fewest threads (256, 512, 1024) whose variant holds n, plan->threads receiving that count; `plan` receives
 // the equivalent register plan.
 int MixedVariant(int n, int k, DescentPlan* plan);
 

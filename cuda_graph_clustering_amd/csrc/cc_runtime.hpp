@@ -162,8 +162,8 @@ struct RunSpec {
   int variant = -1;  // mixed-descent variant when plan.descent is kMixed
 };
 
-// A register-kernel run (ls_kernel auto) moved to the mixed descent of its block size (512 threads up to 4096
-// vertices, else 1024), so that the runs of a batch or a serving lane share one or two launches; same trajectory.
+// A register-kernel run (ls_kernel auto) moved to the mixed descent of its block size (256 threads up to 2048
+// vertices, 512 up to 4096, else 1024), so that the runs of a batch or a serving lane share one or two launches; same trajectory.
 void UseMixedDescent(RunSpec& spec);
 
 struct DescentGroup {

@@ -62,9 +62,9 @@ def test_batch_of_one_equals_solve(cga):
         same(res, dg.solve_pbils(**kw))
 
 
-def test_mixed_batch_spans_both_block_sizes(cga):
-    """Register runs up to 4096 vertices take the 512-thread mixed descent (two workgroups per CU), larger ones the
-    1024-thread one; both in one batch, each run equal to its solo solve."""
+def test_mixed_batch_spans_all_block_sizes(cga):
+    """Register runs up to 2048 vertices take the 256-thread mixed descent (four workgroups per CU), up to 4096 the
+    512-thread one, larger ones the 1024-thread one; all three in one batch, each run equal to its solo solve."""
     shapes = [(1500, 2), (4096, 5), (4097, 8), (4097, 2), (7000, 3), (2500, 7)]
     graphs, params = [], []
     for i, (n, k) in enumerate(shapes):
@@ -75,7 +75,7 @@ def test_mixed_batch_spans_both_block_sizes(cga):
         for (n, _), g, kw, res in zip(shapes, graphs, params, results):
             same(res, g.solve_pbils(**kw))
             assert res["descent_name"] == "mixed"
-            assert res["threads"] == (512 if n <= 4096 else 1024), n
+            assert res["threads"] == (256 if n <= 2048 else 512 if n <= 4096 else 1024), n
     finally:
         for g in graphs:
             g.close()
