@@ -366,12 +366,68 @@ def bench_tags(cga, args):
     print(json.dumps(out))
 
 
+SWEEP_SERIES = {"n128": "graph size, k = 3, pop 128", "pop6000": "population, n = 6000, k = 3",
+                "k128": "clusters, n = 2000, pop 128"}
+
+
+def bench_sweep(cga, args):
+    """The reference's published GPU sweeps (docs/benchmarks.md:127-202, docs/gpu_sweep.sh): ER G(n, 0.33),
+    graph-seed 7, 20 iterations, early stop 1000, gww 0, tournament 5, perturbation 0.4; each point solved with
+    seeds 6, 7 and 8 after a warm-up with seed 5, host graph in (so upload and allocation are inside the time, as
+    on the reference's clock), with our default plan and with ls_kernel = global, the kernel family the reference
+    timed. Every final f must equal the reference's (tests/golden/gpu_trajectories.csv, gpu_f == cpu_f there);
+    the reference's seconds are its RTX 4070 Ti SUPER (CUDA) and Ryzen 7 3700X 16-thread runs. One JSON line."""
+    import csv
+    import statistics
+
+    rows = list(csv.DictReader(open(os.path.join(ROOT, "tests", "golden", "gpu_trajectories.csv"))))
+    points = {}
+    for r in rows:
+        if r["series"] in SWEEP_SERIES:
+            key = (r["series"], int(r["n"]), int(r["k"]), int(r["pop"]))
+            points.setdefault(key, []).append(r)
+    out_points, exact = [], True
+    graphs = {}
+    for (series, n, k, pop), refs in sorted(points.items(), key=lambda kv: (kv[0][0], kv[0][1], kv[0][2], kv[0][3])):
+        if n > args.sweep_max_n:
+            continue
+        if n not in graphs:
+            graphs = {n: cga.erdos_renyi(n, 0.33, 7)}
+        g = graphs[n]
+        common = dict(k=k, population=pop, iterations=20, early_stop=1000, gww=0.0, tournament=5, perturbation=0.4)
+        point = {"series": series, "n": n, "k": k, "pop": pop}
+        for label, ls in (("auto", "auto"), ("global", "global")):
+            cga.solve_pbils(g, seed=5, ls_kernel=ls, **common)
+            times = []
+            for r in sorted(refs, key=lambda r: int(r["seed"])):
+                t0 = time.perf_counter()
+                res = cga.solve_pbils(g, seed=int(r["seed"]), ls_kernel=ls, **common)
+                times.append(time.perf_counter() - t0)
+                exact = exact and res["objective"] == int(r["f"])
+            point[f"ours_{label}_s"] = round(statistics.median(times), 4)
+            if label == "auto":
+                point["descent"] = res["descent_name"]
+        point["reference_gpu_s"] = statistics.median(float(r["gpu_s"]) for r in refs)
+        point["reference_cpu16_s"] = statistics.median(float(r["cpu_s"]) for r in refs)
+        point["vs_reference_gpu"] = round(point["reference_gpu_s"] / point["ours_auto_s"], 1)
+        point["vs_reference_gpu_same_kernel"] = round(point["reference_gpu_s"] / point["ours_global_s"], 1)
+        out_points.append(point)
+        note(f"sweep {series} n={n} k={k} pop={pop}: ours {point['ours_auto_s']} s (global "
+             f"{point['ours_global_s']} s), reference GPU {point['reference_gpu_s']} s")
+    print(json.dumps({"metric": "seconds per 20 PBILS iterations on the reference's published GPU sweeps",
+                      "unit": "s", "higher_is_better": False, "data": "synthetic",
+                      "config": {"workload": "ER G(n, 0.33) graph-seed 7, 20 iterations, gww 0; series: " +
+                                 "; ".join(f"{k}: {v}" for k, v in SWEEP_SERIES.items())},
+                      "bit_exact_vs_reference_f": exact, "points": out_points}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "tags"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "tags", "sweep"])
+    ap.add_argument("--sweep-max-n", type=int, default=32000, help="largest n of --config sweep")
     ap.add_argument("--tags-objects", type=int, default=60000, help="objects of the --config tags file")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
@@ -398,6 +454,9 @@ def main():
 
     if args.config == "tags":
         bench_tags(cga, args)
+        return
+    if args.config == "sweep":
+        bench_sweep(cga, args)
         return
     if args.config == "c3":
         bench_neighborhood(cga, args)

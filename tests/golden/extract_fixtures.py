@@ -5,7 +5,10 @@ Run once in a container that has the reference tree (it reads nothing at test ti
 
 Sources (reference docs/data/*.csv) and the flags that produced them:
   gpu.csv       docs/gpu_sweep.sh:13-16  --density 0.33 --graph-seed 7 --iters 20 --early-stop 1000 --gww 0, row
-                `run=j` used --seed 5+j; gpu_f == cpu_f on every row. Columns kept: series,n,k,pop,seed,f.
+                `run=j` used --seed 5+j; gpu_f == cpu_f on every row. Columns kept: series,n,k,pop,seed,f, plus
+                the reference's measured seconds of the run (gpu_s: cc_gpu --ls-kernel global on its RTX 4070 Ti
+                SUPER, cpu_s: cc_cpu --threads 16 on its Ryzen 7 3700X; docs/benchmarks.md:127-142) as
+                published timings for the sweep comparison in bench.py.
   gww.csv       docs/gww_sweep.py:34-86  --density 0.33 --graph-seed 7|8 --seed 1 --runs 9 --iters I --early-stop I;
                 the record after iteration `iter`; the seed column is the actual --seed. Random graphs only, iter <= 50.
   gww_stop.csv  docs/gww_sweep.py:88-92  defaults (100 iterations, early stop 6); final f and iterations done.
@@ -36,8 +39,9 @@ def main():
     gpu = []
     for r in rows("gpu.csv"):
         assert r["gpu_f"] == r["cpu_f"], r
-        gpu.append(dict(series=r["series"], n=r["n"], k=r["k"], pop=r["pop"], seed=5 + int(r["run"]), f=r["gpu_f"]))
-    write("gpu_trajectories.csv", ["series", "n", "k", "pop", "seed", "f"], gpu)
+        gpu.append(dict(series=r["series"], n=r["n"], k=r["k"], pop=r["pop"], seed=5 + int(r["run"]), f=r["gpu_f"],
+                        gpu_s=r["gpu_s"], cpu_s=r["cpu_s"]))
+    write("gpu_trajectories.csv", ["series", "n", "k", "pop", "seed", "f", "gpu_s", "cpu_s"], gpu)
 
     curves = []
     for r in rows("gww.csv"):
