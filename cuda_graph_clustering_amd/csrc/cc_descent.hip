@@ -328,13 +328,13 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   }
 }
 
-// k in [3, 8], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt.
+// k in [3, 16], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt <= 16.
 //
 // With q_c(v) = s_c - 2 G[v][c], the best move of v goes to the cluster of smallest q_c (ties to the smaller c) and
 // has delta = q_c - q_a + 1 (a = label of v). Per (j, c) one register holds y = -key with
-//   key(j, c) = (q_c << 12) | (j << 6) | (a << 3) | c,
+//   key(j, c) = (q_c << 12) | (j << 8) | (a << 4) | c,
 // and per j the own term qa = q_a << 12. Then z_j = max_c y[j][c] + qa = -x_j with
-//   x_j = ((q_c - q_a) << 12) | (j << 6) | (a << 3) | c,
+//   x_j = ((q_c - q_a) << 12) | (j << 8) | (a << 4) | c,
 // whose signed order is (delta, j, c): the largest z over the thread's vertices is its smallest-delta move, ties to
 // its smaller vertex, and the target comes with it. The scan is thus ceil((k - 1) / 2) v_max3 and one add per
 // vertex. Including c = a in the maximum is harmless: it gives delta 1, never an improving move. The workgroup
@@ -345,15 +345,15 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
 //   y[from] += m,  y[to] -= m        (s_from - 1, s_to + 1, G[u][from] - bit, G[u][to] + bit)
 //   qa += t_a m,   t_a = [a == to] - [a == from]   (fetched per lane with ds_bpermute from a lane table)
 // for every vertex u; the mover then rewrites its own label bits and qa. Lanes beyond n hold y = -2^30, qa = 0 and
-// a label index (63) whose t is 0: their y drifts by at most 4096 n, so they never offer a move.
+// a label index (63) whose t is 0, so they never offer a move.
 template <int T, int K, int kVpt>
 __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   constexpr int kPadY = -(1 << 30);
   constexpr int kPadLabel = 63;
-  static_assert(K >= 3 && K <= 8, "register descent handles k in [3, 8]");
-  static_assert(kVpt <= 63, "vertex slot index must fit 6 bits");
+  static_assert(K >= 3 && K <= 16, "register descent handles k in [3, 16]: labels in 4 bits");
+  static_assert(kVpt <= 16 && kVpt * T <= 32768, "slot index in 4 bits, vertex in 15");
   const int n = r.n;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -385,14 +385,9 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   };
   auto size_add = [&](int c, int64_t d) {  // c uniform
     const uint64_t inc = static_cast<uint64_t>(d) << (16 * (c & 3));
-    if constexpr (kSizeWords == 1) {
-      szp[0] += inc;
-    } else {
-      if (c < 4) {
-        szp[0] += inc;
-      } else {
-        szp[1] += inc;
-      }
+#pragma unroll
+    for (int i = 0; i < kSizeWords; ++i) {
+      if ((c >> 2) == i) szp[i] += inc;  // uniform
     }
   };
   int kc[K];  // (s_c << 12) + c at the start
@@ -414,7 +409,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       const int g = g_in[static_cast<size_t>(c) * n + vc];
-      y[j][c] = real ? (g << 13) - (j << 6) - (a << 3) - kc[c] : kPadY;
+      y[j][c] = real ? (g << 13) - (j << 8) - (a << 4) - kc[c] : kPadY;
       ga = c == a ? g : ga;
       sa = c == a ? size_of(c) : sa;
     }
@@ -436,10 +431,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       for (int c = 1; c < K; ++c) ymax = max(ymax, y[j][c]);
       zmax = max(zmax, ymax + qa[j]);
     }
-    const int x = -zmax;  // ((delta - 1) << 12) | (j << 6) | (a << 3) | c
-    cand[parity][tid] = static_cast<uint8_t>(x & 63);
+    const int x = -zmax;  // ((delta - 1) << 12) | (j << 8) | (a << 4) | c
+    cand[parity][tid] = static_cast<uint8_t>(x & 255);
     if (x >= -4096) return ~0u;  // delta >= 0: no improving move
-    const int j = (x >> 6) & 63;
+    const int j = (x >> 8) & 15;
     return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(tid + j * T);
   };
 
@@ -466,8 +461,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) words[j] = row[j * (T / 32)];
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
-    const int from = (info >> 3) & 7;
-    const int to = info & 7;
+    const int from = (info >> 4) & 15;
+    const int to = info & 15;
     parity ^= 1;
     size_add(to, 1);
     size_add(from, -1);
@@ -493,7 +488,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       }
     }
     // The moved vertex: new label bits in its y, and its own term qa = (s_to << 12) - (G_to << 13), which is
-    // -(y[to] + (j << 6) + (to << 3) + to) once y carries the new sizes and label.
+    // -(y[to] + (j << 8) + (to << 4) + to) once y carries the new sizes and label.
     const int mv_j = v / T;
     if (tid == v - mv_j * T) {
 #pragma unroll
@@ -503,10 +498,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
           int y_to = 0;
 #pragma unroll
           for (int c = 0; c < K; ++c) {
-            y[j][c] -= (to - from) << 3;
+            y[j][c] -= (to - from) << 4;
             if (c == to) y_to = y[j][c];
           }
-          qa[j] = -(y_to + (j << 6) + (to << 3) + to);
+          qa[j] = -(y_to + (j << 8) + (to << 4) + to);
         }
       }
     }
@@ -544,7 +539,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       lab_out[v] = static_cast<uint8_t>(a);
 #pragma unroll
       for (int c = 0; c < K; ++c) {
-        g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>((y[j][c] + kc[c] + (j << 6) + (a << 3)) >> 13);
+        g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>((y[j][c] + kc[c] + (j << 8) + (a << 4)) >> 13);
       }
     }
   }
@@ -1240,7 +1235,7 @@ bool GoK2Any(const Go& go) {
 
 // Register descent instances: T = 256 with kVpt in {1, 2, 4}; T = 1024 with kVpt in {2, 4, 6, 8, 10, 12},
 // bounded by kVpt (K + 5) <= 112 so that the state fits the 128 VGPRs of a 1024-thread workgroup.
-constexpr bool RegsFit(int k, int vpt) { return vpt * (k + 5) <= 112; }
+constexpr bool RegsFit(int k, int vpt) { return vpt * (k + 3) <= 104; }
 
 template <int T, int K, int kVpt>
 bool GoRegsOne(const Go& go) {
@@ -1269,6 +1264,8 @@ bool GoRegsK(const Go& go) {
       case 8: return GoRegsOne<1024, K, 8>(go);
       case 10: return GoRegsOne<1024, K, 10>(go);
       case 12: return GoRegsOne<1024, K, 12>(go);
+      case 14: return GoRegsOne<1024, K, 14>(go);
+      case 16: return GoRegsOne<1024, K, 16>(go);
       default: return false;
     }
   }
@@ -1283,6 +1280,14 @@ bool GoRegs(const Go& go) {
     case 6: return GoRegsK<6>(go);
     case 7: return GoRegsK<7>(go);
     case 8: return GoRegsK<8>(go);
+    case 9: return GoRegsK<9>(go);
+    case 10: return GoRegsK<10>(go);
+    case 11: return GoRegsK<11>(go);
+    case 12: return GoRegsK<12>(go);
+    case 13: return GoRegsK<13>(go);
+    case 14: return GoRegsK<14>(go);
+    case 15: return GoRegsK<15>(go);
+    case 16: return GoRegsK<16>(go);
     default: return false;
   }
 }
@@ -1379,7 +1384,7 @@ int MixedVariant(int n, int k, DescentPlan* plan) {
 }
 
 int RegistersVpt(int n, int k, int* threads) {
-  if (k < 3 || k > 8) return 0;
+  if (k < 3 || k > 16) return 0;
   if (n <= 1024) {
     *threads = 256;
     for (int v : {1, 2, 4}) {
@@ -1387,7 +1392,7 @@ int RegistersVpt(int n, int k, int* threads) {
     }
   }
   *threads = 1024;
-  for (int v : {2, 4, 6, 8, 10, 12}) {
+  for (int v : {2, 4, 6, 8, 10, 12, 14, 16}) {
     if (v * 1024 >= n) return RegsFit(k, v) ? v : 0;
   }
   return 0;

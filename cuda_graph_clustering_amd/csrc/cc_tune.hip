@@ -69,9 +69,9 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
       }
     }
   }
-  if (!wide && k >= 3 && k <= 8) {
+  if (!wide && k >= 3 && k <= 16) {
     for (int t : {256, 1024}) {
-      for (int v : {1, 2, 4, 6, 8, 10, 12}) {
+      for (int v : {1, 2, 4, 6, 8, 10, 12, 14, 16}) {
         if (v * t < n) continue;
         DescentPlan p = base;
         p.descent = Descent::kRegisters;

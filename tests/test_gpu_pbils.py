@@ -51,6 +51,9 @@ CASES = [
     (97, "er", 64, 5, 4, 4, 0.4, 0.2, 9),
     (300, "sbm", 2, 33, 12, 3, 0.4, 0.1, 10),
     (257, "er", 3, 1, 6, 6, 0.4, 0.0, 11),
+    (700, "sbm", 12, 10, 5, 5, 0.4, 0.1, 12),
+    (1500, "er", 16, 8, 4, 4, 0.4, 0.1, 13),
+    (300, "cl", 9, 6, 4, 4, 0.4, 0.2, 14),
 ]
 
 
@@ -261,3 +264,13 @@ def test_register_kernel_descent_longer_than_padding_reset(cga, orc, cpu_threads
     check_same(got, want, "register kernel, long descent")
     assert got["descent_name"] == "registers"
     assert got["accepted_moves"] > 2 * 4096, "the descents must pass the padding reset"
+
+
+def test_register_kernel_sixteen_slots_per_thread(cga, orc, cpu_threads):
+    """n = 16000 at k = 3 takes the register kernel with 16 vertex slots per thread (the widest layout: slot index
+    in 4 bits, vertex in 15); it must follow the oracle exactly."""
+    g = cga.sbm(16000, 3, 0.3, 0.05, 23)
+    kw = dict(k=3, population=2, iterations=2, early_stop=2, seed=6, gww=0.1)
+    got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    check_same(got, want, "register kernel, 16 slots")
+    assert (got["descent_name"], got["threads"]) == ("registers", 1024)
