@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "cc_internal.hpp"
 #include "cc_wave.hpp"
@@ -328,15 +329,22 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   }
 }
 
-// k in [3, 16], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt <= 16.
+// Label field width of the register descent: four bits up to k = 16, five up to k = 32.
+constexpr int RegsLabelBits(int k) { return k <= 16 ? 4 : 5; }
+// Clusters held in registers for a run of k: k itself up to 16, above that k rounded up to a multiple of four; the
+// extra clusters stay empty and are never offered as targets.
+constexpr int RegsClusters(int k) { return k <= 16 ? k : (k + 3) & ~3; }
+
+// k in [3, 32], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt. With b the label
+// field width (RegsLabelBits: 4 bits up to k = 16, 5 above) the slot index j sits at bit 2b and stays below bit 12.
 //
 // With q_c(v) = s_c - 2 G[v][c], the best move of v goes to the cluster of smallest q_c (ties to the smaller c) and
 // has delta = q_c - q_a + 1 (a = label of v). Per (j, c) one register holds y = -key with
-//   key(j, c) = (q_c << 12) | (j << 8) | (a << 4) | c,
+//   key(j, c) = (q_c << 12) | (j << 2b) | (a << b) | c,
 // and per j the own term qa = q_a << 12. Then z_j = max_c y[j][c] + qa = -x_j with
-//   x_j = ((q_c - q_a) << 12) | (j << 8) | (a << 4) | c,
+//   x_j = ((q_c - q_a) << 12) | (j << 2b) | (a << b) | c,
 // whose signed order is (delta, j, c): the largest z over the thread's vertices is its smallest-delta move, ties to
-// its smaller vertex, and the target comes with it. The scan is thus ceil((k - 1) / 2) v_max3 and one add per
+// its smaller vertex, and the target comes with it. The scan is thus ceil((K - 1) / 2) v_max3 and one add per
 // vertex. Including c = a in the maximum is harmless: it gives delta 1, never an improving move. The workgroup
 // minimum of ((delta - 1 + 65535) << 15) | v (32 bits, n <= 32767) is the reference's move order (delta, v); the
 // winner's label and target are read from the owner's entry of an LDS table (cand[]).
@@ -345,30 +353,38 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
 //   y[from] += m,  y[to] -= m        (s_from - 1, s_to + 1, G[u][from] - bit, G[u][to] + bit)
 //   qa += t_a m,   t_a = [a == to] - [a == from]   (fetched per lane with ds_bpermute from a lane table)
 // for every vertex u; the mover then rewrites its own label bits and qa. Lanes beyond n hold y = -2^30, qa = 0 and
-// a label index (63) whose t is 0, so they never offer a move.
+// a label index (63) whose t is 0, so they never offer a move. For k > 16 the K - k empty clusters also hold
+// y = -2^30: no move touches their column, and the mover's label rewrites shift it by (from - to) << b, which sums
+// to the vertex's first label minus its last.
 template <int T, int K, int kVpt>
 __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   constexpr int kPadY = -(1 << 30);
   constexpr int kPadLabel = 63;
-  static_assert(K >= 3 && K <= 16, "register descent handles k in [3, 16]: labels in 4 bits");
-  static_assert(kVpt <= 16 && kVpt * T <= 32768, "slot index in 4 bits, vertex in 15");
+  constexpr int kLb = RegsLabelBits(K);
+  constexpr int kLabelMask = (1 << kLb) - 1;
+  constexpr int kSlotShift = 2 * kLb;
+  using Cand = std::conditional_t<(kSlotShift <= 8), uint8_t, uint16_t>;
+  static_assert(K >= 3 && K <= 32 && RegsClusters(K) == K, "register descent holds 3 to 16 clusters, or 20 to 32 by 4");
+  static_assert((kVpt << kSlotShift) <= 4096 && kVpt * T <= 32768, "slot index below bit 12, vertex in 15 bits");
   const int n = r.n;
+  const int k = r.k;  // == K up to 16; above, K - k empty clusters pad the registers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = WaveIndex();
   __shared__ uint32_t partial[2][kWaves];
-  __shared__ uint8_t cand[2][T];
+  __shared__ Cand cand[2][T];
   __shared__ int32_t src_sh;
   if (tid == 0) {
     src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
   }
   __syncthreads();
   const int src = __builtin_amdgcn_readfirstlane(src_sh);
-  const int16_t* g_in = static_cast<const int16_t*>(r.pop.g) + static_cast<size_t>(src) * K * n;
+  const int16_t* g_in = static_cast<const int16_t*>(r.pop.g) + static_cast<size_t>(src) * k * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
-  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * K;
+  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * k;
+  auto live = [&](int c) { return K <= 16 || c < k; };  // c known at compile time; k uniform
 
   // Cluster sizes (< 2^15 on the narrow path) as 16-bit fields, four per 64-bit word, wave-uniform: a move updates
   // them with scalar shifts and adds.
@@ -378,7 +394,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   for (int i = 0; i < kSizeWords; ++i) szp[i] = 0;
 #pragma unroll
   for (int c = 0; c < K; ++c) {
-    szp[c >> 2] |= static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(sizes_in[c])) << (16 * (c & 3));
+    const int size = live(c) ? sizes_in[c] : 0;
+    szp[c >> 2] |= static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(size)) << (16 * (c & 3));
   }
   auto size_of = [&](int c) {  // c known at compile time
     return static_cast<int>((szp[c >> 2] >> (16 * (c & 3))) & 0xFFFFu);
@@ -408,8 +425,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     int ga = 0, sa = 0;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-      const int g = g_in[static_cast<size_t>(c) * n + vc];
-      y[j][c] = real ? (g << 13) - (j << 8) - (a << 4) - kc[c] : kPadY;
+      const int g = live(c) ? g_in[static_cast<size_t>(c) * n + vc] : 0;
+      y[j][c] = real && live(c) ? (g << 13) - (j << kSlotShift) - (a << kLb) - kc[c] : kPadY;
       ga = c == a ? g : ga;
       sa = c == a ? size_of(c) : sa;
     }
@@ -431,10 +448,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       for (int c = 1; c < K; ++c) ymax = max(ymax, y[j][c]);
       zmax = max(zmax, ymax + qa[j]);
     }
-    const int x = -zmax;  // ((delta - 1) << 12) | (j << 8) | (a << 4) | c
-    cand[parity][tid] = static_cast<uint8_t>(x & 255);
+    const int x = -zmax;  // ((delta - 1) << 12) | (j << 2b) | (a << b) | c
+    cand[parity][tid] = static_cast<Cand>(x & ((1 << kSlotShift) - 1));
     if (x >= -4096) return ~0u;  // delta >= 0: no improving move
-    const int j = (x >> 8) & 15;
+    const int j = (x >> kSlotShift) & ((4096 >> kSlotShift) - 1);
     return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(tid + j * T);
   };
 
@@ -461,8 +478,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) words[j] = row[j * (T / 32)];
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
-    const int from = (info >> 4) & 15;
-    const int to = info & 15;
+    const int from = (info >> kLb) & kLabelMask;
+    const int to = info & kLabelMask;
     parity ^= 1;
     size_add(to, 1);
     size_add(from, -1);
@@ -488,7 +505,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       }
     }
     // The moved vertex: new label bits in its y, and its own term qa = (s_to << 12) - (G_to << 13), which is
-    // -(y[to] + (j << 8) + (to << 4) + to) once y carries the new sizes and label.
+    // -(y[to] + (j << 2b) + (to << b) + to) once y carries the new sizes and label.
     const int mv_j = v / T;
     if (tid == v - mv_j * T) {
 #pragma unroll
@@ -498,10 +515,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
           int y_to = 0;
 #pragma unroll
           for (int c = 0; c < K; ++c) {
-            y[j][c] -= (to - from) << 4;
+            y[j][c] -= (to - from) << kLb;
             if (c == to) y_to = y[j][c];
           }
-          qa[j] = -(y_to + (j << 8) + (to << 4) + to);
+          qa[j] = -(y_to + (j << kSlotShift) + (to << kLb) + to);
         }
       }
     }
@@ -528,7 +545,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   }
 
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
-  int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * K * n;
+  int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * k * n;
 #pragma unroll
   for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
 #pragma unroll
@@ -539,15 +556,18 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       lab_out[v] = static_cast<uint8_t>(a);
 #pragma unroll
       for (int c = 0; c < K; ++c) {
-        g_out[static_cast<size_t>(c) * n + v] = static_cast<int16_t>((y[j][c] + kc[c] + (j << 8) + (a << 4)) >> 13);
+        if (live(c)) {
+          g_out[static_cast<size_t>(c) * n + v] =
+              static_cast<int16_t>((y[j][c] + kc[c] + (j << kSlotShift) + (a << kLb)) >> 13);
+        }
       }
     }
   }
-  if (tid < K) {
+  if (tid < k) {
     int size = 0;
 #pragma unroll
     for (int c = 0; c < K; ++c) size = tid == c ? size_of(c) : size;
-    r.next.sizes[static_cast<size_t>(s) * K + tid] = size;
+    r.next.sizes[static_cast<size_t>(s) * k + tid] = size;
   }
   if (tid == 0) {
     r.next.f[s] = objective;
@@ -1235,7 +1255,11 @@ bool GoK2Any(const Go& go) {
 
 // Register descent instances: T = 256 with kVpt in {1, 2, 4}; T = 1024 with kVpt in {2, 4, 6, 8, 10, 12},
 // bounded by kVpt (K + 5) <= 112 so that the state fits the 128 VGPRs of a 1024-thread workgroup.
-constexpr bool RegsFit(int k, int vpt) { return vpt * (k + 3) <= 104; }
+// Registers of a register-descent thread: vpt (K + 3) for the slot state, within the 128 VGPRs of 16 waves per CU;
+// the slot index stays below bit 12 of the key.
+constexpr bool RegsFit(int clusters, int vpt) {
+  return vpt * (clusters + 3) <= 104 && (vpt << (2 * RegsLabelBits(clusters))) <= 4096;
+}
 
 template <int T, int K, int kVpt>
 bool GoRegsOne(const Go& go) {
@@ -1288,6 +1312,10 @@ bool GoRegs(const Go& go) {
     case 14: return GoRegsK<14>(go);
     case 15: return GoRegsK<15>(go);
     case 16: return GoRegsK<16>(go);
+    case 17: case 18: case 19: case 20: return GoRegsK<20>(go);
+    case 21: case 22: case 23: case 24: return GoRegsK<24>(go);
+    case 25: case 26: case 27: case 28: return GoRegsK<28>(go);
+    case 29: case 30: case 31: case 32: return GoRegsK<32>(go);
     default: return false;
   }
 }
@@ -1384,16 +1412,20 @@ int MixedVariant(int n, int k, DescentPlan* plan) {
 }
 
 int RegistersVpt(int n, int k, int* threads) {
-  if (k < 3 || k > 16) return 0;
+  if (k < 3 || k > 32) return 0;
+  const int clusters = RegsClusters(k);
   if (n <= 1024) {
-    *threads = 256;
     for (int v : {1, 2, 4}) {
-      if (v * 256 >= n) return v;
+      if (v * 256 >= n) {
+        if (!RegsFit(clusters, v)) break;
+        *threads = 256;
+        return v;
+      }
     }
   }
   *threads = 1024;
   for (int v : {2, 4, 6, 8, 10, 12, 14, 16}) {
-    if (v * 1024 >= n) return RegsFit(k, v) ? v : 0;
+    if (v * 1024 >= n) return RegsFit(clusters, v) ? v : 0;
   }
   return 0;
 }

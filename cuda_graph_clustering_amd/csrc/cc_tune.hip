@@ -69,7 +69,7 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
       }
     }
   }
-  if (!wide && k >= 3 && k <= 16) {
+  if (!wide && k >= 3 && k <= 32) {
     for (int t : {256, 1024}) {
       for (int v : {1, 2, 4, 6, 8, 10, 12, 14, 16}) {
         if (v * t < n) continue;

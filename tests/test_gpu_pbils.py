@@ -54,6 +54,11 @@ CASES = [
     (700, "sbm", 12, 10, 5, 5, 0.4, 0.1, 12),
     (1500, "er", 16, 8, 4, 4, 0.4, 0.1, 13),
     (300, "cl", 9, 6, 4, 4, 0.4, 0.2, 14),
+    (400, "er", 17, 8, 4, 4, 0.4, 0.1, 15),
+    (900, "sbm", 20, 6, 4, 4, 0.4, 0.1, 16),
+    (2000, "er", 24, 6, 3, 3, 0.4, 0.1, 17),
+    (1100, "cl", 30, 5, 3, 3, 0.4, 0.2, 18),
+    (300, "sbm", 28, 6, 4, 4, 0.4, 0.1, 19),
 ]
 
 
@@ -161,6 +166,22 @@ def test_golden_gpu_csv_rows(cga, cpu_threads, ls_kernel):
         got = cga.solve_pbils(g, k=3, population=128, iterations=20, early_stop=1000, gww=0.0, seed=int(r["seed"]),
                               ls_kernel=ls_kernel)
         assert got["objective"] == int(r["f"])
+
+
+def test_golden_gpu_csv_cluster_series(cga):
+    """The reference's published k sweep (docs/benchmarks.md, ER G(2000, 0.33) graph-seed 7, pop 128, 20
+    iterations, gww 0, seeds 6-8, k from 2 to 32): every final f of the default plan equals the reference's."""
+    import csv
+    import os
+
+    rows = [r for r in csv.DictReader(open(os.path.join(os.path.dirname(__file__), "golden", "gpu_trajectories.csv")))
+            if r["series"] == "k128"]
+    assert len({r["k"] for r in rows}) >= 9
+    g = cga.erdos_renyi(2000, 0.33, 7)
+    for r in rows:
+        got = cga.solve_pbils(g, k=int(r["k"]), population=128, iterations=20, early_stop=1000, gww=0.0,
+                              seed=int(r["seed"]))
+        assert got["objective"] == int(r["f"]), (r["k"], r["seed"])
 
 
 def test_golden_gww_curves(cga):
@@ -274,3 +295,17 @@ def test_register_kernel_sixteen_slots_per_thread(cga, orc, cpu_threads):
     got, want = run_both(cga, orc, g, cpu_threads, **kw)
     check_same(got, want, "register kernel, 16 slots")
     assert (got["descent_name"], got["threads"]) == ("registers", 1024)
+
+
+@pytest.mark.parametrize("n,k,threads", [(400, 17, 256), (900, 20, 256), (2000, 24, 1024), (600, 32, 1024),
+                                         (300, 28, 256), (2048, 31, 1024)])
+def test_register_kernel_above_sixteen_clusters(cga, orc, cpu_threads, n, k, threads):
+    """k in [17, 32] runs the register kernel with five-bit label fields and k rounded up to a multiple of four
+    registers per slot (the extra clusters stay empty); it must follow the oracle exactly, and the empty clusters
+    must never appear among the labels."""
+    g = cga.erdos_renyi(n, 0.33, k)
+    kw = dict(k=k, population=4, iterations=3, early_stop=3, seed=k + 1, gww=0.1)
+    got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    check_same(got, want, f"register kernel, n={n} k={k}")
+    assert (got["descent_name"], got["threads"]) == ("registers", threads)
+    assert got["labels"].max() < k
