@@ -66,7 +66,7 @@ def make_graph(cga, gen):
     return cga.sbm(gen[1], gen[2], gen[3], gen[4], gen[5])
 
 
-REGISTER_DESCENTS = ("registers", "k2-registers", "generic-lds")
+REGISTER_DESCENTS = ("registers", "k2-registers", "k3-packed", "generic-lds")
 
 
 def descent_bytes_per_move(n, k, descent, mean_degree):

@@ -19,7 +19,7 @@ LS_AUTO, LS_GLOBAL, LS_SHARED, LS_WIDE = 0, 1, 2, 3
 LS_KERNELS = {"auto": LS_AUTO, "global": LS_GLOBAL, "shared": LS_SHARED, "wide": LS_WIDE}
 REBUILDS = {"popcount": 0, "mfma": 1}
 DESCENTS = {0: "generic-global", 1: "generic-lds", 2: "k2-registers", 3: "k2-wide", 4: "k3-wide", 5: "registers",
-            6: "mixed", 7: "k3-chunk"}
+            6: "mixed", 7: "k3-chunk", 8: "k3-packed"}
 
 OK, EINVAL, ENOMEM, EDEVICE, ENODEV = 0, 1, 2, 3, 4
 

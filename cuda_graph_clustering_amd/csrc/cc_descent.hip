@@ -575,6 +575,166 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   }
 }
 
+// k = 3, narrow path up to n = 32767, per-vertex state packed in one pair of registers:
+//   lo = G0 | G1 << 16,  hi = G2 | a << 16        (G_c = neighbours of v in cluster c < 2^15, a = label of v)
+// Thread t owns the kVpt consecutive vertices v = t kVpt + j, so the row bits of a move that it needs are one
+// 64-bit window of the row (two dword loads and one v_alignbit), however many vertices it holds. A move adds
+// d = [c == to] - [c == from] to field c of every neighbour: one masked add per register.
+//
+// The scan recomputes the register kernel's key from the counts, with y'_c = (G_c << 13) - ((s_c << 12) + c):
+// the largest y'_c is the best target (smallest q_c = s_c - 2 G_c, ties to the smaller c), and
+//   z = max_c y'_c - y'_a - 5 a - (j << 4) = -(((q_c - q_a) << 12) | (j << 4) | (a << 2) | c),
+// so the largest z over the thread's vertices is its smallest-delta move, ties to its smaller vertex. Vertices
+// beyond n hold G = (24576, 4096, 4096) with a = 0: their y'_0 exceeds the other two by more than any size
+// difference, so z = -(j << 4) never offers a move. Their row bits are not masked (they come from the zero tail or
+// the next row), so their counts drift by at most one per move; they are reset every 1024 moves.
+template <int T, int kVpt>
+__device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
+  constexpr int kWaves = T / kWave;
+  constexpr uint32_t kPadLo = 24576u | (4096u << 16);
+  constexpr uint32_t kPadHi = 4096u;
+  static_assert(kVpt >= 1 && kVpt <= 32 && kVpt * T <= 32768, "slot index in 5 bits, vertex in 15 bits");
+  Ctl* ctl = r.ctl;
+  const int n = r.n;
+  const int tid = threadIdx.x;
+  __shared__ uint32_t partial[2][kWaves];
+  __shared__ uint8_t cand[2][T];
+  __shared__ int32_t src_sh;
+  if (tid == 0) {
+    src_sh = r.select ? Tournament(r.select_stream, s, ctl->iteration, r.pop.f, r.population, r.tournament) : s;
+  }
+  __syncthreads();
+  const int src = __builtin_amdgcn_readfirstlane(src_sh);
+  const int16_t* g_in = static_cast<const int16_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
+  const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
+  const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
+  int size0 = __builtin_amdgcn_readfirstlane(sizes_in[0]);
+  int size1 = __builtin_amdgcn_readfirstlane(sizes_in[1]);
+  int size2 = __builtin_amdgcn_readfirstlane(sizes_in[2]);
+
+  const int first = tid * kVpt;
+  uint32_t lo[kVpt], hi[kVpt];
+#pragma unroll
+  for (int j = 0; j < kVpt; ++j) {
+    const int v = first + j;
+    const bool real = v < n;
+    const int vc = real ? v : n - 1;
+    const uint32_t g0 = static_cast<uint16_t>(g_in[vc]);
+    const uint32_t g1 = static_cast<uint16_t>(g_in[static_cast<size_t>(n) + vc]);
+    const uint32_t g2 = static_cast<uint16_t>(g_in[2 * static_cast<size_t>(n) + vc]);
+    const uint32_t a = lab_in[vc];
+    lo[j] = real ? g0 | (g1 << 16) : kPadLo;
+    hi[j] = real ? g2 | (a << 16) : kPadHi;
+  }
+  const int64_t f_start = Uniform(r.pop.f[src]);
+  int64_t objective = f_start;
+  int64_t applied = 0;
+  const int64_t cap = MoveCap(f_start, n);
+
+  // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
+  auto scan = [&](int parity) {
+    const int y0c = -(size0 << 12);
+    const int y1c = -((size1 << 12) + 1);
+    const int y2c = -((size2 << 12) + 2);
+    int zmax = -(1 << 30);
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const int y0 = static_cast<int>((lo[j] & 0xFFFFu) << 13) + y0c;
+      const int y1 = static_cast<int>((lo[j] >> 16) << 13) + y1c;
+      const int y2 = static_cast<int>((hi[j] & 0xFFFFu) << 13) + y2c;
+      const int a = static_cast<int>(hi[j] >> 16);
+      const int ya = a == 0 ? y0 : (a == 1 ? y1 : y2);
+      const int z = max(max(y0, y1), y2) - ya - 5 * a - (j << 4);
+      zmax = max(zmax, z);
+    }
+    const int x = -zmax;  // ((delta - 1) << 12) | (j << 4) | (a << 2) | c
+    cand[parity][tid] = static_cast<uint8_t>(x & 15);
+    if (x >= -4096) return ~0u;  // delta >= 0: no improving move
+    return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(first + ((x >> 4) & 31));
+  };
+
+  int parity = 0;
+  uint32_t key = scan(parity);
+  const int row_words = __builtin_amdgcn_readfirstlane(r.words);
+  const uint64_t* const row_base =
+      reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
+  const int window = first >> 5;  // the dword holding bit `first` of a row; the window's second dword follows
+  const uint32_t shift = static_cast<uint32_t>(first & 31);
+  while (true) {
+    const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
+    if (best == ~0u) break;
+    const int delta = static_cast<int>(best >> 15) - 65535 + 1;
+    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
+    // Bits [first, first + kVpt) of the row lie in its dwords window and window + 1 (first & 31 + kVpt <= 63);
+    // for the last rows the second dword may lie in the zero tail after the last row (kRowTailBytes).
+    const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
+    const uint32_t w0 = row[0];
+    const uint32_t w1 = row[1];
+    const int owner = v / kVpt;
+    const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
+    const int from = (info >> 2) & 3;
+    const int to = info & 3;
+    parity ^= 1;
+    size0 += (to == 0) - (from == 0);
+    size1 += (to == 1) - (from == 1);
+    size2 += (to == 2) - (from == 2);
+    // per-field increments of a neighbour, uniform; no field leaves [0, 2^15), so the halves never carry
+    const uint32_t dlo = static_cast<uint32_t>((to == 0) - (from == 0)) + (static_cast<uint32_t>((to == 1) - (from == 1)) << 16);
+    const uint32_t dhi = static_cast<uint32_t>((to == 2) - (from == 2));
+    const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const uint32_t mask = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1));
+      lo[j] += mask & dlo;
+      hi[j] += mask & dhi;
+    }
+    // the moved vertex: new label (its own counts do not change: it is not its own neighbour)
+    const int mv_j = v - owner * kVpt;
+    if (tid == owner) {
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) {
+        if (j == mv_j) hi[j] = (hi[j] & 0xFFFFu) | (static_cast<uint32_t>(to) << 16);
+      }
+    }
+    if ((applied & 1023) == 1023) {
+#pragma unroll
+      for (int j = 0; j < kVpt; ++j) {
+        if (first + j >= n) {
+          lo[j] = kPadLo;
+          hi[j] = kPadHi;
+        }
+      }
+    }
+    key = scan(parity);
+    objective += delta;
+    if (++applied >= cap) {
+      if (tid == 0) {
+        ctl->error = 1;
+        ctl->stopped = 1;
+      }
+      break;
+    }
+  }
+
+  uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
+  int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
+#pragma unroll
+  for (int j = 0; j < kVpt; ++j) {
+    const int v = first + j;
+    if (v < n) {
+      lab_out[v] = static_cast<uint8_t>(hi[j] >> 16);
+      g_out[v] = static_cast<int16_t>(lo[j] & 0xFFFFu);
+      g_out[static_cast<size_t>(n) + v] = static_cast<int16_t>(lo[j] >> 16);
+      g_out[2 * static_cast<size_t>(n) + v] = static_cast<int16_t>(hi[j] & 0xFFFFu);
+    }
+  }
+  if (tid < 3) r.next.sizes[static_cast<size_t>(s) * 3 + tid] = tid == 0 ? size0 : (tid == 1 ? size1 : size2);
+  if (tid == 0) {
+    r.next.f[s] = objective;
+    r.moves[s] = applied;
+  }
+}
+
 // Vertices per thread whose loads a wide-path scan keeps in flight per group.
 constexpr int kWideUnroll = 8;
 
@@ -1089,6 +1249,11 @@ __global__ void __launch_bounds__(T) KernelDescentRegs(const RunDev* runs, ItemL
   Persistent(runs, list, [](const RunDev& r, int s) { DescentRegsBody<T, K, kVpt>(r, s); });
 }
 
+template <int T, int kVpt>
+__global__ void __launch_bounds__(T) KernelDescentK3Packed(const RunDev* runs, ItemList list) {
+  Persistent(runs, list, [](const RunDev& r, int s) { DescentK3PackedBody<T, kVpt>(r, s); });
+}
+
 template <int T>
 __global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, ItemList list) {
   Persistent(runs, list, [](const RunDev& r, int s) { DescentK2WideBody<T>(r, s); });
@@ -1327,6 +1492,17 @@ bool GoWide(const Go& go) {
   return false;
 }
 
+bool GoK3Packed(const Go& go) {
+  switch (go.plan->vpt) {
+    case 16: return go.Run(KernelDescentK3Packed<1024, 16>, 1024, 0);
+    case 20: return go.Run(KernelDescentK3Packed<1024, 20>, 1024, 0);
+    case 24: return go.Run(KernelDescentK3Packed<1024, 24>, 1024, 0);
+    case 28: return go.Run(KernelDescentK3Packed<1024, 28>, 1024, 0);
+    case 32: return go.Run(KernelDescentK3Packed<1024, 32>, 1024, 0);
+    default: return false;
+  }
+}
+
 template <int T>
 struct WideK2 {
   static constexpr auto kernel = KernelDescentK2Wide<T>;
@@ -1350,6 +1526,8 @@ bool Dispatch(const Go& go) {
       return p.wide && p.k == 2 && GoWide<WideK2>(go);
     case Descent::kK3Wide:
       return p.wide && p.k == 3 && GoWide<WideK3>(go);
+    case Descent::kK3Packed:
+      return !p.wide && p.k == 3 && p.threads == 1024 && GoK3Packed(go);
     case Descent::kK3Chunk:
       if (!p.wide || p.k != 3 || p.lds_bytes == 0) return false;
       if (p.threads == 256) return go.Run(KernelDescentK3Chunk<256>, 256, p.lds_bytes);
@@ -1409,6 +1587,13 @@ int MixedVariant(int n, int k, DescentPlan* plan) {
   if (smallest >= 0) return smallest;
   const int small = pick(kMixedTable512, kMixedCount512, kMixedThreads512);
   return small >= 0 ? small : pick(kMixedTable, kMixedCount, kMixedThreads);
+}
+
+int K3PackedVpt(int n) {
+  for (int v : {16, 20, 24, 28, 32}) {
+    if (v * 1024 >= n) return v;
+  }
+  return 0;
 }
 
 int RegistersVpt(int n, int k, int* threads) {

@@ -139,9 +139,10 @@ enum class Descent : int {
   kK2Registers = 2,
   kK2Wide = 3,
   kK3Wide = 4,
-  kRegisters = 5,  // k in [3, 8], narrow, state in registers
+  kRegisters = 5,  // k in [3, 32], narrow, state in registers
   kMixed = 6,      // a batch launch over runs of different register kernels (256/512/1024 threads), by RunDev::variant
   kK3Chunk = 7,    // k = 3 wide, packed words plus per-chunk pair summaries in LDS (K3ChunkLdsBytes)
+  kK3Packed = 8,   // k = 3 narrow, packed counts in registers, consecutive vertices per thread (n up to 32767)
 };
 
 struct DescentPlan {
@@ -168,6 +169,9 @@ size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 size_t K3ChunkLdsBytes(int n);
 int RegistersVpt(int n, int k, int* threads);
+// Vertices per thread of the packed k = 3 descent (1024 threads) for n <= 32767: the smallest of 16, 20, ..., 32
+// that covers n.
+int K3PackedVpt(int n);
 // The mixed-descent variant that runs a register plan of (k, n), or -1 if none: an index into the table of the launch
 // with the fewest threads (256, 512, 1024) whose variant holds n, plan->threads receiving that count; `plan` receives
 // the equivalent register plan.

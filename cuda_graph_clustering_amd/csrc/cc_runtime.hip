@@ -253,6 +253,12 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
       plan.vpt = vpt;
       return plan;
     }
+    if (k == 3) {  // beyond the register kernel's reach: two registers per vertex
+      plan.descent = Descent::kK3Packed;
+      plan.threads = 1024;
+      plan.vpt = K3PackedVpt(n);
+      return plan;
+    }
   }
   if (p.ls_kernel == CC_LS_AUTO && k == 2 && !wide) {
     plan.descent = Descent::kK2Registers;
@@ -659,9 +665,10 @@ const char* DescentName(Descent d) {
     case Descent::kK2Registers: return "k = 2, registers";
     case Descent::kK2Wide: return "k = 2, packed words";
     case Descent::kK3Wide: return "k = 3, packed words";
-    case Descent::kRegisters: return "k = 3..8, registers";
+    case Descent::kRegisters: return "k = 3..32, registers";
     case Descent::kMixed: return "mixed register kernels";
     case Descent::kK3Chunk: return "k = 3, packed words with chunk summaries";
+    case Descent::kK3Packed: return "k = 3, packed counts in registers";
   }
   return "?";
 }

@@ -84,6 +84,13 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
       }
     }
   }
+  if (!wide && k == 3 && n > 8192) {
+    DescentPlan p = base;
+    p.descent = Descent::kK3Packed;
+    p.threads = 1024;
+    p.vpt = K3PackedVpt(n);
+    if (p.vpt > 0 && DescentSupported(p)) out.push_back(p);
+  }
   if (wide && (k == 2 || k == 3)) {
     std::vector<int> resident = {0};
     for (int b : {population / 2, population / 4}) {
@@ -228,7 +235,7 @@ int LoadTuned(const char* path) {
     p.wide = wide;
     p.blocks = blocks;
     // any single-run kernel (the mixed batch kernel is chosen per batch, not per shape)
-    const bool valid = descent >= 0 && descent <= static_cast<int>(Descent::kK3Chunk) &&
+    const bool valid = descent >= 0 && descent <= static_cast<int>(Descent::kK3Packed) &&
                        p.descent != Descent::kMixed && lds <= DescentLdsLimit() && DescentSupported(p);
     if (!valid) {
       std::fclose(f);

@@ -89,9 +89,10 @@ typedef struct cc_pbils_stats {
 #define CC_DESCENT_K2_REGISTERS 2
 #define CC_DESCENT_K2_WIDE 3
 #define CC_DESCENT_K3_WIDE 4
-#define CC_DESCENT_REGISTERS 5 /* k in [3, 8], n <= 32767: state in VGPRs */
+#define CC_DESCENT_REGISTERS 5 /* k in [3, 32], n up to 16384 at k = 3 (fewer at larger k): state in VGPRs */
 #define CC_DESCENT_MIXED 6     /* batched: one launch over runs of different register kernels */
 #define CC_DESCENT_K3_CHUNK 7  /* k = 3 wide: packed words plus per-chunk move summaries in LDS */
+#define CC_DESCENT_K3_PACKED 8 /* k = 3, n <= 32767: three 16-bit counts and the label per vertex in two VGPRs */
 
 void cc_pbils_default_params(cc_pbils_params* params);
 

@@ -309,3 +309,15 @@ def test_register_kernel_above_sixteen_clusters(cga, orc, cpu_threads, n, k, thr
     check_same(got, want, f"register kernel, n={n} k={k}")
     assert (got["descent_name"], got["threads"]) == ("registers", threads)
     assert got["labels"].max() < k
+
+
+@pytest.mark.parametrize("n,vpt", [(16385, 20), (24000, 24), (32767, 32)])
+def test_packed_k3_kernel(cga, orc, cpu_threads, n, vpt):
+    """k = 3 beyond the register kernel's 16384 vertices runs the packed kernel (three 16-bit counts and the label
+    in two registers per vertex, consecutive vertices per thread) up to the narrow path's last n; it must follow the
+    oracle exactly, with the smallest slot count that covers n (16385: almost a whole slot of padding)."""
+    g = cga.sbm(n, 3, 0.3, 0.05, 29)
+    kw = dict(k=3, population=2, iterations=2, early_stop=2, seed=7, gww=0.1)
+    got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    check_same(got, want, f"packed k = 3, n={n}")
+    assert (got["descent_name"], got["threads"], got["vertices_per_thread"]) == ("k3-packed", 1024, vpt)

@@ -73,3 +73,17 @@ def test_table_round_trip(cga, tmp_path):
         again = dg.solve_pbils(**kw)
     assert (again["descent_name"], again["threads"]) == (tuned["descent_name"], tuned["threads"])
     assert again["objective"] == tuned["objective"] and (again["labels"] == tuned["labels"]).all()
+
+
+def test_k3_shape_tries_registers_and_packed(cga, orc, cpu_threads):
+    """Above 8192 vertices at k = 3 the tuner times the packed kernel beside the register kernel; both follow the
+    same trajectory."""
+    g = cga.erdos_renyi(10000, 0.2, 4)
+    kw = dict(k=3, population=4, iterations=2, early_stop=2, gww=0.1, seed=3)
+    with cga.DeviceGraph(g) as dg:
+        cands, chosen = cga.autotune(dg, tune_iterations=1, **kw)
+        assert {c["descent"] for c in cands} >= {"registers", "k3-packed"}
+        assert len({c["accepted_moves"] for c in cands}) == 1
+        res = dg.solve_pbils(**kw)
+    assert res["descent_name"] == cands[chosen]["descent"]
+    check_oracle(cga, orc, g, res, cpu_threads, **kw)
