@@ -329,14 +329,16 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   }
 }
 
-// Label field width of the register descent: four bits up to k = 16, five up to k = 32.
-constexpr int RegsLabelBits(int k) { return k <= 16 ? 4 : 5; }
+// Label field width of the register descent: two bits up to k = 4, four up to k = 16, five up to k = 32.
+constexpr int RegsLabelBits(int k) { return k <= 4 ? 2 : k <= 16 ? 4 : 5; }
 // Clusters held in registers for a run of k: k itself up to 16, above that k rounded up to a multiple of four; the
 // extra clusters stay empty and are never offered as targets.
 constexpr int RegsClusters(int k) { return k <= 16 ? k : (k + 3) & ~3; }
 
-// k in [3, 32], narrow path, per-vertex state in registers. Thread t owns v = t + j T, j < kVpt. With b the label
-// field width (RegsLabelBits: 4 bits up to k = 16, 5 above) the slot index j sits at bit 2b and stays below bit 12.
+// k in [3, 32], narrow path, per-vertex state in registers. Thread t owns the kVpt consecutive vertices
+// v = t kVpt + j, so the row bits of a move that it needs are one 64-bit window of the row (two dword loads and one
+// v_alignbit) and a wave's loads share one or two cache lines. With b the label field width (RegsLabelBits: 2 bits
+// up to k = 4, 4 up to 16, 5 above) the slot index j sits at bit 2b and stays below bit 12.
 //
 // With q_c(v) = s_c - 2 G[v][c], the best move of v goes to the cluster of smallest q_c (ties to the smaller c) and
 // has delta = q_c - q_a + 1 (a = label of v). Per (j, c) one register holds y = -key with
@@ -368,11 +370,11 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   using Cand = std::conditional_t<(kSlotShift <= 8), uint8_t, uint16_t>;
   static_assert(K >= 3 && K <= 32 && RegsClusters(K) == K, "register descent holds 3 to 16 clusters, or 20 to 32 by 4");
   static_assert((kVpt << kSlotShift) <= 4096 && kVpt * T <= 32768, "slot index below bit 12, vertex in 15 bits");
+  static_assert(kVpt <= 32, "a thread's row bits lie in one 64-bit window");
   const int n = r.n;
   const int k = r.k;  // == K up to 16; above, K - k empty clusters pad the registers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = WaveIndex();
   __shared__ uint32_t partial[2][kWaves];
   __shared__ Cand cand[2][T];
   __shared__ int32_t src_sh;
@@ -416,9 +418,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   int lab4[kVpt];  // label * 4: the ds_bpermute address of the lane table
   // Branch-free: a padding slot (v >= n) loads vertex n - 1 and keeps its padding values (branches around writes
   // to the slot arrays make the compiler copy whole arrays between blocks).
+  const int first = tid * kVpt;
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
-    const int v = tid + j * T;
+    const int v = first + j;
     const bool real = v < n;
     const int vc = real ? v : n - 1;
     const int a = lab_in[vc];
@@ -452,7 +455,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     cand[parity][tid] = static_cast<Cand>(x & ((1 << kSlotShift) - 1));
     if (x >= -4096) return ~0u;  // delta >= 0: no improving move
     const int j = (x >> kSlotShift) & ((4096 >> kSlotShift) - 1);
-    return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(tid + j * T);
+    return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(first + j);
   };
 
   int parity = 0;
@@ -460,24 +463,21 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
-  const int half = lane >> 5;
+  const int window = first >> 5;  // the dword holding bit `first` of a row; the window's second dword follows
+  const uint32_t shift = static_cast<uint32_t>(first & 31);
   while (true) {
     const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
-    // The row loads go out before the LDS read of the winner's labels, so the two latencies overlap; every load is
-    // issued (a slot beyond n reads word 0 and discards it), so no branch separates them. They are vector loads
-    // (scalar loads here would be serialised by the kernel's SGPR pressure) of one 32-bit half word per lane: lanes
-    // 0-31 take the low half of the slot's 64-bit row word, lanes 32-63 the high half. Slot j is T / 32 half words
-    // after slot j - 1, so all loads share one address register; slots beyond n read into the zero tail after the
-    // last row (kRowTailBytes) or the next row, and are discarded.
-    const GlobalWords32 row =
-        (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + (wave * kWave >> 5) + half;
-    uint32_t words[kVpt];
-#pragma unroll
-    for (int j = 0; j < kVpt; ++j) words[j] = row[j * (T / 32)];
-    const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][v & (T - 1)]));
+    // The row loads go out before the LDS read of the winner's labels, so the two latencies overlap. Bits
+    // [first, first + kVpt) of the row lie in its dwords window and window + 1 ((first & 31) + kVpt <= 63); for the
+    // last rows the second dword may lie in the zero tail after the last row (kRowTailBytes).
+    const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
+    const uint32_t w0 = row[0];
+    const uint32_t w1 = row[1];
+    const int owner = v / kVpt;
+    const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> kLb) & kLabelMask;
     const int to = info & kLabelMask;
     parity ^= 1;
@@ -485,11 +485,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     size_add(from, -1);
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
+    const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     int m[kVpt];
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
       // m = 4096 - 8192 bit, from the sign-extended one-bit field (0 or -1)
-      m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(words[j]), lane & 31, 1) << 13) + 4096;
+      m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1) << 13) + 4096;
       const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
       qa[j] += __mul24(ta, m[j]);  // |ta| <= 1, |m| = 4096: a full-rate 24-bit multiply
     }
@@ -506,8 +507,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     }
     // The moved vertex: new label bits in its y, and its own term qa = (s_to << 12) - (G_to << 13), which is
     // -(y[to] + (j << 2b) + (to << b) + to) once y carries the new sizes and label.
-    const int mv_j = v / T;
-    if (tid == v - mv_j * T) {
+    const int mv_j = v - owner * kVpt;
+    if (tid == owner) {
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
         if (j == mv_j) {
@@ -527,7 +528,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     if ((applied & 4095) == 4095) {
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
-        if (tid + j * T >= n) {
+        if (first + j >= n) {
 #pragma unroll
           for (int c = 0; c < K; ++c) y[j][c] = kPadY;
         }
@@ -550,7 +551,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
-    const int v = tid + j * T;
+    const int v = first + j;
     if (v < n) {
       const int a = lab4[j] >> 2;
       lab_out[v] = static_cast<uint8_t>(a);
