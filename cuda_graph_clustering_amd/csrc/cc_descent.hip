@@ -174,7 +174,8 @@ using GlobalWords32 = const __attribute__((address_space(1))) uint32_t*;
 // k = 2, narrow path (n <= 32767). The key of a move is (delta + 2^15) << 16 | v << 1 | label, a 32-bit minimum
 // (delta of an improving move lies in [-(n - 1), -1], so it fits).
 //
-// Thread t owns v_j = t + j T. Per slot two VGPRs hold
+// Thread t owns the kVpt consecutive vertices v_j = t kVpt + j (so the row bits of a move that it needs are one 64-bit
+// window). Per slot two VGPRs hold
 //   key_j  ((d_j + 2^15) << 16) | (v_j << 1) | label, where d_j is the delta of moving v_j to the other cluster:
 //          sv - 2 e (label 0) or 2 - sv + 2 e (label 1), with e = G1 - G0 and sv = n1 - n0 + 1. |d_j| <= n - 1,
 //          so key_j is the packed key of v_j's move, and the move improves exactly when key_j < 2^31;
@@ -189,10 +190,10 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   static_assert(kVpt * T <= 32768, "v_j << 1 | label must fit in 16 bits");
+  static_assert(kVpt <= 32, "a thread's row bits lie in one 64-bit window");
   const int n = r.n;
   const int words = r.words;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
   const int wave = WaveIndex();
   __shared__ uint32_t partial[2][kWaves];
   __shared__ int32_t src_sh;
@@ -221,7 +222,7 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     const int sv = n - 2 * n0 + 1;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int v = tid + j * T;
+      const int v = tid * kVpt + j;
       const bool real = v < n;
       const int vc = real ? v : n - 1;
       const int e = static_cast<int>(g_in[n + vc]) - static_cast<int>(g_in[vc]);
@@ -242,7 +243,8 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   const int row_words = __builtin_amdgcn_readfirstlane(words);
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
-  const int half = lane >> 5;
+  const int window = (tid * kVpt) >> 5;  // the dword holding the thread's first bit; the second dword follows
+  const uint32_t shift = static_cast<uint32_t>((tid * kVpt) & 31);
   uint32_t mine = scan();
   int64_t applied = 0;
   const int64_t cap = MoveCap(objective, n);
@@ -254,41 +256,39 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     const int delta = static_cast<int>(best >> 16) - 32768;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>((best & 0xFFFFu) >> 1));
     const int from = static_cast<int>(best & 1u);
-    // Row words of slot j: lanes 0-31 read the low half of the 64-bit word that holds bits base .. base + 63, lanes
-    // 32-63 the high half. All loads are issued before any is used. A slot beyond n reads the next row or the zero
-    // tail after the last row (kRowTailBytes; at most kVpt T / 8 <= 4096 bytes past the row's start) and its bit is
-    // ignored (sg_j = 0).
-    uint32_t w[kVpt];
+    // Bits [t kVpt, t kVpt + kVpt) of the row lie in its dwords `window` and `window` + 1. In LDS the second index
+    // is clamped to the row (a dword past the row holds only bits of slots beyond n); in global memory it may lie in
+    // the next row or the zero tail after the last row (kRowTailBytes; at most kVpt T / 8 + 4 bytes past the row's
+    // start). Bits of slots beyond n are ignored (sg_j = 0).
+    uint32_t w0, w1;
     if constexpr (kRowsLds) {
       const uint32_t* row = reinterpret_cast<const uint32_t*>(rows_lds + static_cast<size_t>(v) * row_words);
-#pragma unroll
-      for (int j = 0; j < kVpt; ++j) {
-        const int idx = min((wave * kWave + j * T) >> 6, row_words - 1);  // wave-uniform, stays in the row
-        w[j] = row[2 * idx + half];
-      }
+      w0 = row[min(window, 2 * row_words - 1)];
+      w1 = row[min(window + 1, 2 * row_words - 1)];
     } else {
-      const GlobalWords32 row =
-          (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + (wave * kWave >> 5) + half;
-#pragma unroll
-      for (int j = 0; j < kVpt; ++j) w[j] = row[j * (T / 32)];
+      const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
+      w0 = row[0];
+      w1 = row[1];
     }
+    const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     n0 += from ? 1 : -1;
     const int step = from ? -2 : 2;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
-      const int bit = __builtin_amdgcn_sbfe(static_cast<int>(w[j]), lane & 31, 1);  // 0 or -1
+      const int bit = __builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1);  // 0 or -1
       const int t = __mul24(bit, 2 * step) + step;                                  // step (1 - 2 A[v][v_j])
       key[j] += static_cast<uint32_t>(__mul24(sg[j], t));
     }
     // The mover: its key becomes the reverse move's (delta -> -delta, label 1 - from). Only the wave that owns it
     // takes the (uniform) branch.
-    if (wave == ((v & (T - 1)) >> 6)) {
-      const int jv = v / T;  // uniform
+    const int owner = v / kVpt;  // uniform
+    if (wave == (owner >> 6)) {
+      const int jv = v - owner * kVpt;
       const uint32_t flipped = (static_cast<uint32_t>(32768 - delta) << 16) | (static_cast<uint32_t>(v) << 1) |
                                static_cast<uint32_t>(1 - from);
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
-        if (j == jv && tid == v - j * T) {
+        if (j == jv && tid == owner) {
           key[j] = flipped;
           sg[j] = from ? 65536 : -65536;
         }
@@ -310,7 +310,7 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   const int sv = n - 2 * n0 + 1;
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
-    const int v = tid + j * T;
+    const int v = tid * kVpt + j;
     if (v < n) {
       const int label = static_cast<int>(key[j] & 1u);
       const int d = static_cast<int>(key[j] >> 16) - 32768;

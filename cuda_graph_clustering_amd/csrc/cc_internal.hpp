@@ -17,10 +17,9 @@ constexpr int kWave = 64;
 constexpr int kMaxClusters = 64;
 constexpr int kMaxVerticesNarrow = 32767;
 constexpr int kMaxVerticesWide = 1 << 20;
-// Zero bytes after the last adjacency row of a device graph. The register descents read row words for all their
-// vertex slots and ignore the bits of slots beyond n; those reads stay within vpt T / 8 bytes of the row's start for
-// k = 2 (at most 32768 / 8 = 4096), and within vpt T / 8 + 4 bytes for k >= 3 (the 64-bit window of a thread's
-// consecutive vertices: at most 4100 bytes, of which a row holds at least 8).
+// Zero bytes after the last adjacency row of a device graph. The register descents read a 64-bit window of the row
+// at each thread's first vertex and ignore the bits of slots beyond n; those reads stay within vpt T / 8 + 4 bytes of
+// the row's start (at most 32768 / 8 + 4 = 4100, of which a row holds at least 8).
 constexpr size_t kRowTailBytes = 4096;
 
 // Run state that the iteration kernels read and advance on the device, so that no host round trip is needed per
