@@ -111,15 +111,16 @@ def wide_graph(cga):
     return cga.chung_lu(40000, 2.5, 16.0, 9)
 
 
-@pytest.mark.parametrize("k,ls_kernel", [(2, "auto"), (3, "auto"), (3, "global"), (4, "auto")])
+@pytest.mark.parametrize("k,ls_kernel", [(2, "auto"), (3, "auto"), (3, "global"), (4, "auto"), (13, "auto")])
 def test_wide_path_full_size_trajectory(cga, orc, cpu_threads, wide_graph, k, ls_kernel):
     """n above the 16-bit range: int32 G, the packed k = 2 kernel and the k = 3 chunk-summary kernel (auto) or the
-    generic global kernel. (The streaming k = 3 kernel runs beside the others in the autotuner test.)"""
+    generic global kernel, whose wide scan loads the counts eight clusters at a time (k = 13: a full and a partial
+    group). (The streaming k = 3 kernel runs beside the others in the autotuner test.)"""
     got, want = run_both(cga, orc, wide_graph, cpu_threads, k=k, population=4, iterations=2, early_stop=2,
                          gww=0.25, seed=3, ls_kernel=ls_kernel)
     check_same(got, want, f"wide k={k} {ls_kernel}")
     expected = {(2, "auto"): "k2-wide", (3, "auto"): "k3-chunk", (3, "global"): "generic-global",
-                (4, "auto"): "generic-global"}[(k, ls_kernel)]
+                (4, "auto"): "generic-global", (13, "auto"): "generic-global"}[(k, ls_kernel)]
     assert got["descent_name"] == expected
 
 
