@@ -326,12 +326,12 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     n0 += from ? 1 : -1;
     const int step = from ? -2 : 2;
-#pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      const int bit = __builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1);  // 0 or -1
-      const int t = __mul24(bit, 2 * step) + step;                                  // step (1 - 2 A[v][v_j])
+    Unroll<kVpt>([&](auto slot) {
+      constexpr int j = decltype(slot)::value;
+      const int bit = SignedBit<j>(bits);           // 0 or -1
+      const int t = __mul24(bit, 2 * step) + step;  // step (1 - 2 A[v][v_j])
       key[j] += static_cast<uint32_t>(__mul24(sg[j], t));
-    }
+    });
     // The mover: its key becomes the reverse move's (delta -> -delta, label 1 - from). Only the wave that owns it
     // takes the (uniform) branch.
     const int owner = v / kVpt;  // uniform
@@ -380,6 +380,16 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     r.next.f[s] = objective;
     r.moves[s] = applied;
   }
+}
+
+// The register descent's move term of vertex slot J, m = 4096 - 8192 A[v][slot] = (bit J of `bits`, sign-extended,
+// & -8192) | 4096, as v_bfe_i32 and one v_and_or_b32. gfx950's three-operand encoding takes no literal, so the mask
+// comes in a VGPR and 4096 in an SGPR (left to itself the compiler spends four instructions).
+template <int J>
+__device__ __forceinline__ int MoveTerm(uint32_t bits, int mask, int bias) {
+  int r;
+  asm("v_bfe_i32 %0, %1, %2, 1\n\tv_and_or_b32 %0, %0, %3, %4" : "=&v"(r) : "v"(bits), "n"(J), "v"(mask), "s"(bias));
+  return r;
 }
 
 // Label field width of the register descent: two bits up to k = 4, four up to k = 16, five up to k = 32.
@@ -540,13 +550,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     const int table = (lane == to) - (lane == from);
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     int m[kVpt];
-#pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      // m = 4096 - 8192 bit, from the sign-extended one-bit field (0 or -1)
-      m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1) << 13) + 4096;
+    Unroll<kVpt>([&](auto slot) {
+      constexpr int j = decltype(slot)::value;
+      m[j] = MoveTerm<j>(bits, ~8191, 4096);
       const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
       qa[j] += __mul24(ta, m[j]);  // |ta| <= 1, |m| = 4096: a full-rate 24-bit multiply
-    }
+    });
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       if (c == from) {
@@ -736,12 +745,12 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     const uint32_t dlo = static_cast<uint32_t>((to == 0) - (from == 0)) + (static_cast<uint32_t>((to == 1) - (from == 1)) << 16);
     const uint32_t dhi = static_cast<uint32_t>((to == 2) - (from == 2));
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
-#pragma unroll
-    for (int j = 0; j < kVpt; ++j) {
-      const uint32_t mask = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1));
+    Unroll<kVpt>([&](auto slot) {
+      constexpr int j = decltype(slot)::value;
+      const uint32_t mask = static_cast<uint32_t>(SignedBit<j>(bits));
       lo[j] += mask & dlo;
       hi[j] += mask & dhi;
-    }
+    });
     // the moved vertex: new label (its own counts do not change: it is not its own neighbour)
     const int mv_j = v - owner * kVpt;
     if (tid == owner) {

@@ -10,8 +10,31 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 namespace cc::hip {
+
+// Bit J of x, sign-extended (0 or -1), as one v_bfe_i32. With a constant offset the compiler turns the sbfe builtin
+// into a shift pair, and folding that into what follows costs one to two more instructions per use; the register
+// descents take this once per vertex slot and move.
+template <int J>
+__device__ __forceinline__ int SignedBit(uint32_t x) {
+  static_assert(J >= 0 && J < 32, "bit index");
+  int r;
+  asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(r) : "v"(x), "n"(J));
+  return r;
+}
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N - 1>{}): a loop whose index is a
+// compile-time constant in the body.
+template <typename F, int... I>
+__device__ __forceinline__ void UnrollImpl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void Unroll(F&& f) {
+  UnrollImpl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int kCtrl>
 __device__ __forceinline__ uint32_t DppMove(uint32_t x) {
