@@ -528,8 +528,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int window = first >> 5;  // the dword holding bit `first` of a row; the window's second dword follows
   const uint32_t shift = static_cast<uint32_t>(first & 31);
+  uint32_t warm = 0;  // the previous move's cache-warming load, consumed one move later
   while (true) {
-    const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
+    const uint32_t wave_best = WaveMin(key);
+    const uint32_t best = BlockMinOfWaves<kWaves>(wave_best, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
@@ -539,6 +541,13 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
     const uint32_t w0 = row[0];
     const uint32_t w1 = row[1];
+    // The next winner is often one of this move's runners-up: each wave touches the cache lines of its own best
+    // candidate's row (one dword per 128-byte line; the winner's wave re-reads the winner's row). Issued after the
+    // row loads, because vector loads return in order: an earlier one would hold up the wait for the row. The value
+    // is consumed, without effect, one move later, when it has long arrived.
+    const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
+    const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+    const uint32_t warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> kLb) & kLabelMask;
@@ -549,6 +558,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
+    asm volatile("" ::"v"(warm));
+    warm = warm_next;
     int m[kVpt];
     Unroll<kVpt>([&](auto slot) {
       constexpr int j = decltype(slot)::value;

@@ -97,12 +97,11 @@ __device__ __forceinline__ T WaveMin(T x) {
   return Min(Min(ReadLane(x, 0), ReadLane(x, 16)), Min(ReadLane(x, 32), ReadLane(x, 48)));
 }
 
-// Minimum over a workgroup of kWaves waves (kWaves <= 16), one barrier. `partial` is the half of a double buffer
-// of kWaves entries that this move uses.
+// Minimum over a workgroup of kWaves waves (kWaves <= 16) from each wave's own minimum, one barrier. `partial` is
+// the half of a double buffer of kWaves entries that this move uses.
 template <int kWaves, typename T>
-__device__ __forceinline__ T BlockMin(T key, T* partial) {
+__device__ __forceinline__ T BlockMinOfWaves(T key, T* partial) {
   static_assert(kWaves >= 1 && kWaves <= 16 && (kWaves & (kWaves - 1)) == 0, "waves per block: power of two <= 16");
-  key = WaveMin(key);
   if constexpr (kWaves == 1) {
     return key;
   } else {
@@ -113,6 +112,11 @@ __device__ __forceinline__ T BlockMin(T key, T* partial) {
     x = RowMin(x);
     return ReadLane(x, 0);
   }
+}
+
+template <int kWaves, typename T>
+__device__ __forceinline__ T BlockMin(T key, T* partial) {
+  return BlockMinOfWaves<kWaves>(WaveMin(key), partial);
 }
 
 // A value the same in every lane, moved into scalar registers.
