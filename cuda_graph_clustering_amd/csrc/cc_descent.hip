@@ -734,8 +734,11 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int window = first >> 5;  // the dword holding bit `first` of a row; the window's second dword follows
   const uint32_t shift = static_cast<uint32_t>(first & 31);
+  const int lane = tid & 63;
+  uint32_t warm = 0;  // the previous move's cache-warming load, consumed one move later
   while (true) {
-    const uint32_t best = BlockMin<kWaves>(key, partial[parity]);
+    const uint32_t wave_best = WaveMin(key);
+    const uint32_t best = BlockMinOfWaves<kWaves>(wave_best, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
@@ -744,6 +747,10 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
     const uint32_t w0 = row[0];
     const uint32_t w1 = row[1];
+    // warm the lines of this wave's best candidate's row, as in the register kernel
+    const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
+    const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+    const uint32_t warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> 2) & 3;
@@ -756,6 +763,8 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     const uint32_t dlo = static_cast<uint32_t>((to == 0) - (from == 0)) + (static_cast<uint32_t>((to == 1) - (from == 1)) << 16);
     const uint32_t dhi = static_cast<uint32_t>((to == 2) - (from == 2));
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
+    asm volatile("" ::"v"(warm));
+    warm = warm_next;
     Unroll<kVpt>([&](auto slot) {
       constexpr int j = decltype(slot)::value;
       const uint32_t mask = static_cast<uint32_t>(SignedBit<j>(bits));
