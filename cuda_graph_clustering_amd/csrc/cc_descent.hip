@@ -427,6 +427,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   constexpr int kWaves = T / kWave;
   constexpr int kPadY = -(1 << 30);
   constexpr int kPadLabel = 63;
+  constexpr bool kWarm = K >= 5;  // cache warming of the candidates' rows (below)
   constexpr int kLb = RegsLabelBits(K);
   constexpr int kLabelMask = (1 << kLb) - 1;
   constexpr int kSlotShift = 2 * kLb;
@@ -544,10 +545,15 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     // The next winner is often one of this move's runners-up: each wave touches the cache lines of its own best
     // candidate's row (one dword per 128-byte line; the winner's wave re-reads the winner's row). Issued after the
     // row loads, because vector loads return in order: an earlier one would hold up the wait for the row. The value
-    // is consumed, without effect, one move later, when it has long arrived.
-    const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
-    const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
-    const uint32_t warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
+    // is consumed, without effect, one move later, when it has long arrived. Measured: 2.7 % faster at k = 5 and
+    // n = 10000, 6 % at k >= 12, but 2-8 % slower at k <= 4, whose moves are short enough for the extra address
+    // work to outweigh the cache hits; so from k = 5 on.
+    uint32_t warm_next = 0;
+    if constexpr (kWarm) {
+      const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
+      const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+      warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
+    }
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> kLb) & kLabelMask;
@@ -558,8 +564,10 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
-    asm volatile("" ::"v"(warm));
-    warm = warm_next;
+    if constexpr (kWarm) {
+      asm volatile("" ::"v"(warm));
+      warm = warm_next;
+    }
     int m[kVpt];
     Unroll<kVpt>([&](auto slot) {
       constexpr int j = decltype(slot)::value;
