@@ -326,12 +326,14 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     n0 += from ? 1 : -1;
     const int step = from ? -2 : 2;
-    Unroll<kVpt>([&](auto slot) {
-      constexpr int j = decltype(slot)::value;
-      const int bit = SignedBit<j>(bits);           // 0 or -1
-      const int t = __mul24(bit, 2 * step) + step;  // step (1 - 2 A[v][v_j])
+    // The builtin, not SignedBit: here the compiler folds its shift form into one v_mad_i32_i24 per slot, and an
+    // opaque v_bfe_i32 made it fall back to a quarter-rate 32-bit multiply (11 % slower at C3).
+#pragma unroll
+    for (int j = 0; j < kVpt; ++j) {
+      const int bit = __builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1);  // 0 or -1
+      const int t = __mul24(bit, 2 * step) + step;                         // step (1 - 2 A[v][v_j])
       key[j] += static_cast<uint32_t>(__mul24(sg[j], t));
-    });
+    }
     // The mover: its key becomes the reverse move's (delta -> -delta, label 1 - from). Only the wave that owns it
     // takes the (uniform) branch.
     const int owner = v / kVpt;  // uniform
