@@ -3,7 +3,8 @@
  *
  * Plain C: POD structs, raw pointers and sizes, integer status codes. The C++ entry points of the reference
  * (include/cc_gpu.hpp:12-20 of the reference) are re-declared in cc_gpu.hpp over this ABI; any other host language
- * binds these functions directly (see INTEGRATION.md for a ctypes and a JNI-style binding).
+ * binds these functions directly (a ctypes, cgo or JNI stub over these declarations; the Python package of this
+ * library is such a ctypes binding).
  *
  * Graph layout everywhere: n rows of words64(n) = max(1, ceil(n/64)) uint64 words, row-major; bit j of row i is
  * bit (j % 64) of word j / 64. The matrix is symmetric with a zero diagonal and zero padding bits.
