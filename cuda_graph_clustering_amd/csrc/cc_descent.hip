@@ -534,6 +534,18 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   uint32_t warm = 0;  // the previous move's cache-warming load, consumed one move later
   while (true) {
     const uint32_t wave_best = WaveMin(key);
+    // Each wave touches the cache lines of its own best candidate's row (one dword per 128-byte line) before the
+    // barrier: the winner is one of these candidates, so its row is on its way while the workgroup minimum is
+    // formed, and the runners-up are often the next moves' winners. The value is consumed, without effect, one move
+    // later. Vector loads return in order, so the wait for the row below also covers this load, which was issued
+    // earlier. Measured at k = 5, n = 10000: 5.85 ms per launch without, 5.71 with the load issued after the row's,
+    // 5.59 with it here.
+    uint32_t warm_next = 0;
+    if constexpr (kWarm) {
+      const int cand_v = wave_best == ~0u ? 0 : static_cast<int>(wave_best & 0x7FFFu);
+      const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+      warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
+    }
     const uint32_t best = BlockMinOfWaves<kWaves>(wave_best, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
@@ -544,18 +556,6 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
     const uint32_t w0 = row[0];
     const uint32_t w1 = row[1];
-    // The next winner is often one of this move's runners-up: each wave touches the cache lines of its own best
-    // candidate's row (one dword per 128-byte line; the winner's wave re-reads the winner's row). Issued after the
-    // row loads, because vector loads return in order: an earlier one would hold up the wait for the row. The value
-    // is consumed, without effect, one move later, when it has long arrived. Measured: 2.7 % faster at k = 5 and
-    // n = 10000, 6 % at k >= 12, but 2-8 % slower at k <= 4, whose moves are short enough for the extra address
-    // work to outweigh the cache hits; so from k = 5 on.
-    uint32_t warm_next = 0;
-    if constexpr (kWarm) {
-      const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
-      const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
-      warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
-    }
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> kLb) & kLabelMask;
