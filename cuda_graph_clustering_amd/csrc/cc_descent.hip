@@ -429,7 +429,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   constexpr int kWaves = T / kWave;
   constexpr int kPadY = -(1 << 30);
   constexpr int kPadLabel = 63;
-  constexpr bool kWarm = K >= 5;  // cache warming of the candidates' rows (below)
+  // Cache warming of the candidates' rows (below): before the barrier for k in [5, 11], after the winner's row loads
+  // from k = 12 on, none up to k = 4. Measured at n = 2000 and, for k = 5, at n = 10000: before the barrier is 2.1 %
+  // faster than after the row at k = 5, after the row 9 % faster than before the barrier at k >= 12, and either
+  // placement is 2-6 % slower than none at k <= 4, whose moves are too short for the extra loads to pay.
+  constexpr bool kWarmEarly = K >= 5 && K < 12;
+  constexpr bool kWarmLate = K >= 12;
   constexpr int kLb = RegsLabelBits(K);
   constexpr int kLabelMask = (1 << kLb) - 1;
   constexpr int kSlotShift = 2 * kLb;
@@ -541,7 +546,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     // earlier. Measured at k = 5, n = 10000: 5.85 ms per launch without, 5.71 with the load issued after the row's,
     // 5.59 with it here.
     uint32_t warm_next = 0;
-    if constexpr (kWarm) {
+    if constexpr (kWarmEarly) {
       const int cand_v = wave_best == ~0u ? 0 : static_cast<int>(wave_best & 0x7FFFu);
       const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
       warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
@@ -556,6 +561,11 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
     const uint32_t w0 = row[0];
     const uint32_t w1 = row[1];
+    if constexpr (kWarmLate) {  // after the row loads, so that the wait for the row does not cover it
+      const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
+      const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+      warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
+    }
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> kLb) & kLabelMask;
@@ -566,7 +576,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     // lane c of the table holds t_c = [c == to] - [c == from]
     const int table = (lane == to) - (lane == from);
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
-    if constexpr (kWarm) {
+    if constexpr (kWarmEarly || kWarmLate) {
       asm volatile("" ::"v"(warm));
       warm = warm_next;
     }
