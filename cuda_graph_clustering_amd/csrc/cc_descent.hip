@@ -765,6 +765,11 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
   uint32_t warm = 0;  // the previous move's cache-warming load, consumed one move later
   while (true) {
     const uint32_t wave_best = WaveMin(key);
+    // warm the lines of this wave's best candidate's row before the barrier, as in the register kernel (at
+    // n = 24000 1.4 % faster than after the row loads)
+    const int cand_v = wave_best == ~0u ? 0 : static_cast<int>(wave_best & 0x7FFFu);
+    const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
+    const uint32_t warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     const uint32_t best = BlockMinOfWaves<kWaves>(wave_best, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
@@ -774,10 +779,6 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words) + window;
     const uint32_t w0 = row[0];
     const uint32_t w1 = row[1];
-    // warm the lines of this wave's best candidate's row, as in the register kernel
-    const int cand_v = wave_best == ~0u ? v : static_cast<int>(wave_best & 0x7FFFu);
-    const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
-    const uint32_t warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     const int owner = v / kVpt;
     const int info = __builtin_amdgcn_readfirstlane(static_cast<int>(cand[parity][owner]));
     const int from = (info >> 2) & 3;
