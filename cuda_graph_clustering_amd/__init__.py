@@ -347,6 +347,7 @@ class DeviceGraph:
     """A graph uploaded once to the GPU; solves on it start with the bitset already in HBM."""
 
     def __init__(self, graph):
+        self.graph = graph  # the host bitset it was uploaded from
         self.n = graph.n
         self.edges = graph.edges
         handle = ctypes.c_void_p()

@@ -91,7 +91,7 @@ __device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
 
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
   int parity = 0;
   int prev_from = -1, prev_to = -1;  // the last move, not yet folded into sz[parity]
   int mv_v = -1, mv_from = 0, mv_to = 0;
@@ -300,7 +300,7 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   const uint32_t shift = static_cast<uint32_t>((tid * kVpt) & 31);
   uint32_t mine = scan();
   int64_t applied = 0;
-  const int64_t cap = MoveCap(objective, n);
+  const int64_t cap = MoveCap(objective);
   int parity = 0;
   while (true) {
     const uint32_t best = BlockMin<kWaves>(mine, partial[parity]);
@@ -510,7 +510,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
 
   // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
   auto scan = [&](int parity) {
@@ -730,7 +730,7 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
   const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
 
   // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
   auto scan = [&](int parity) {
@@ -942,7 +942,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   uint64_t key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
   int parity = 0;
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
@@ -1063,7 +1063,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   uint64_t key = key_of();
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
   int parity = 0;
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
@@ -1244,7 +1244,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   uint64_t key = candidate();
   int64_t objective = f_start;
   int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start, n);
+  const int64_t cap = MoveCap(f_start);
   int parity = 0;
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
@@ -1329,19 +1329,30 @@ constexpr size_t kStaticLdsReserve = 2048;
 // A persistent workgroup: claims items (run, slot) from the launch's queue until none is left and runs the descent
 // of each; items of stopped runs are skipped. The barrier before each claim keeps the shared state of the previous
 // item alive until every wave is done with it.
+//
+// Another workgroup of the same run may set ctl->stopped in the middle of this launch (a descent that reaches its
+// move cap), so the flag is read once, by the thread that claims the item, and handed to the whole workgroup through
+// LDS: every wave then takes the same branch and meets the same barriers.
 template <typename Body>
 __device__ __forceinline__ void Persistent(const RunDev* runs, const ItemList& list, Body&& body) {
   __shared__ int32_t claimed;
+  __shared__ int32_t skip;
   while (true) {
     __syncthreads();
-    if (threadIdx.x == 0) claimed = atomicAdd(list.queue, 1);
+    if (threadIdx.x == 0) {
+      const int index = atomicAdd(list.queue, 1);
+      claimed = index;
+      skip = index < list.count
+                 ? __hip_atomic_load(&runs[list.items[index].run].ctl->stopped, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT)
+                 : 1;
+    }
     __syncthreads();
     const int index = claimed;
     if (index >= list.count) break;
+    if (skip) continue;
     const Item it = list.items[index];
-    const RunDev& r = runs[it.run];
-    if (r.ctl->stopped) continue;
-    body(r, it.slot);
+    body(runs[it.run], it.slot);
   }
 }
 
@@ -1463,22 +1474,25 @@ void AllowLds(Kernel kernel, size_t bytes) {
   }
 }
 
-// Launch context shared by the dispatchers below; `launch` false only asks whether the instance exists.
+// What a dispatch does with the kernel instance it finds: nothing (does the instance exist?), set its dynamic-LDS
+// attribute only (outside a stream capture, before the launches are recorded), or set the attribute and launch.
+enum class GoMode { kProbe, kAttributes, kLaunch };
+
+// Launch context shared by the dispatchers below.
 struct Go {
   const RunDev* runs;
   const ItemList* list;
   const DescentPlan* plan;
   hipStream_t stream;
-  bool launch;
+  GoMode mode;
 
   int Blocks() const { return plan->blocks > 0 ? plan->blocks : list->count; }
 
   template <typename Kernel>
   bool Run(Kernel kernel, int threads, size_t lds) const {
-    if (launch) {
-      AllowLds(kernel, lds);
-      kernel<<<Blocks(), threads, lds, stream>>>(runs, *list);
-    }
+    if (mode == GoMode::kProbe) return true;
+    AllowLds(kernel, lds);
+    if (mode == GoMode::kLaunch) kernel<<<Blocks(), threads, lds, stream>>>(runs, *list);
     return true;
   }
 };
@@ -1660,12 +1674,17 @@ size_t DescentLdsLimit() { return kLdsBytes - kStaticLdsReserve; }
 
 bool DescentSupported(const DescentPlan& plan) {
   const ItemList none{nullptr, 0, nullptr};
-  return Dispatch(Go{nullptr, &none, &plan, nullptr, false});
+  return Dispatch(Go{nullptr, &none, &plan, nullptr, GoMode::kProbe});
+}
+
+void SetDescentAttributes(const DescentPlan& plan) {
+  const ItemList none{nullptr, 0, nullptr};
+  Dispatch(Go{nullptr, &none, &plan, nullptr, GoMode::kAttributes});
 }
 
 void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan& plan, hipStream_t stream) {
   if (items.count == 0) return;
-  Dispatch(Go{runs, &items, &plan, stream, true});
+  Dispatch(Go{runs, &items, &plan, stream, GoMode::kLaunch});
 }
 
 size_t GenericLdsBytes(int n, int k, int wide) {

@@ -86,13 +86,11 @@ struct ItemList {
   int32_t* queue;  // descent: next item to claim (reset in-stream by KernelResetQueues)
 };
 
-// Every accepted move lowers f by at least 1, so a descent ends after at most f moves; in practice it takes well
-// under n moves. A descent that reaches this cap means a broken invariant: it stops, sets ctl->error and the run
-// fails instead of spinning on the device.
-__host__ __device__ inline int64_t MoveCap(int64_t f_start, int n) {
-  const int64_t practical = 64 * static_cast<int64_t>(n) + 4096;
-  return f_start < practical ? f_start + 1 : practical;
-}
+// Every accepted move lowers f by at least 1 and f never goes below 0, so a descent from f_start ends after at most
+// f_start moves, exactly as the uncapped descent of src/cc_pbils_cpu.cpp:141-149 does. The cap f_start + 1 can
+// therefore only be reached through a broken invariant (a wrong delta or G): the descent then stops, sets
+// ctl->error, and the run fails instead of spinning on the device. No valid descent is cut short.
+__host__ __device__ inline int64_t MoveCap(int64_t f_start) { return f_start + 1; }
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 
@@ -164,6 +162,9 @@ inline bool SameLaunch(const DescentPlan& a, const DescentPlan& b) {
 
 // A persistent launch: plan.blocks workgroups claim the items of `items` through items.queue.
 void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan& plan, hipStream_t stream);
+// Sets the dynamic-LDS attribute of the plan's kernel instance without launching it (LaunchDescent sets it too): a
+// caller that records launches in a stream capture calls this first, so no attribute call happens inside the capture.
+void SetDescentAttributes(const DescentPlan& plan);
 bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);

@@ -564,10 +564,7 @@ void LaunchSet::EnqueueIteration(Timer* timer, cc_pbils_stats* stats) {
 }
 
 void LaunchSet::SetLdsAttributes() {
-  for (const DescentGroup& d : descent_) {
-    const ItemList none{d.list.items, 0, d.list.queue};
-    LaunchDescent(runs_, none, d.plan, stream_);  // no items: only the kernel attributes are set
-  }
+  for (const DescentGroup& d : descent_) SetDescentAttributes(d.plan);
 }
 
 const Ctl* LaunchSet::PullCtl() {
@@ -627,6 +624,9 @@ Batch::Batch(const std::vector<RunSpec>& specs, hipStream_t stream) : stream_(st
 }
 
 Batch::~Batch() {
+  // The executable graph's kernel nodes point into the run states and launch tables that the members release after
+  // this body; every replay on the stream ends before any of them goes, also on an error path that left work queued.
+  (void)hipStreamSynchronize(stream_);
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
 }

@@ -130,6 +130,10 @@ class Oracle:
         lib.orc_local_search_labels.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                                 ctypes.c_int, _i32p, _i64p]
         lib.orc_local_search_labels.restype = ctypes.c_int64
+        lib.orc_best_move_labels.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                             _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_int)]
+        lib.orc_best_move_labels.restype = ctypes.c_int
         lib.orc_rank_slots.argtypes = [_i64p, ctypes.c_int, _i32p]
         lib.orc_winner_copies.argtypes = [ctypes.c_double, ctypes.c_int]
         lib.orc_winner_copies.restype = ctypes.c_int
@@ -205,6 +209,14 @@ class Oracle:
         f = self.lib.orc_local_search_labels(_ptr(g.bits, _u64p), g.n, g.words, g.edges, k, _ptr(labels, _i32p),
                                              ctypes.byref(moves))
         return f, labels, moves.value
+
+    def best_move(self, g, k, labels, plain=False):
+        """(delta, v, to) of the steepest move from `labels`; plain=True uses the literal reference scan."""
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        v, to = ctypes.c_int(0), ctypes.c_int(0)
+        d = self.lib.orc_best_move_labels(_ptr(g.bits, _u64p), g.n, g.words, g.edges, k, _ptr(labels, _i32p),
+                                          int(plain), ctypes.byref(v), ctypes.byref(to))
+        return d, v.value, to.value
 
     def rank_slots(self, f):
         f = np.ascontiguousarray(f, dtype=np.int64)

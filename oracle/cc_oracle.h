@@ -74,6 +74,7 @@ void orc_state_free(orc_state* s);
 void orc_state_copy(orc_state* dst, const orc_state* src);
 void orc_rebuild(orc_state* s, const uint64_t* bits, uint32_t words, uint64_t edges);
 int orc_best_move(const orc_state* s, int* out_v, int* out_to);
+int orc_best_move_plain(const orc_state* s, int* out_v, int* out_to);
 void orc_apply_move(orc_state* s, const uint64_t* bits, uint32_t words, int v, int to);
 int64_t orc_local_search(orc_state* s, const uint64_t* bits, uint32_t words, int64_t* accepted_moves);
 void orc_perturb(orc_state* s, const uint64_t* bits, uint32_t words, uint64_t edges, double probability,
@@ -83,6 +84,8 @@ void orc_perturb(orc_state* s, const uint64_t* bits, uint32_t words, uint64_t ed
 int64_t orc_rebuild_labels(const uint64_t* bits, uint32_t n, uint32_t words, uint64_t edges, int k,
                            const int32_t* labels, int32_t* g_out, int32_t* sizes_out);
 /* Local search from given labels; returns f, writes final labels and the number of accepted moves. */
+int orc_best_move_labels(const uint64_t* bits, uint32_t n, uint32_t words, uint64_t edges, int k,
+                         const int32_t* labels, int plain, int* v, int* to);
 int64_t orc_local_search_labels(const uint64_t* bits, uint32_t n, uint32_t words, uint64_t edges, int k,
                                 int32_t* labels, int64_t* moves);
 

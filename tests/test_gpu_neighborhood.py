@@ -83,3 +83,24 @@ def test_invalid_input_rejected(cga):
             with pytest.raises(cga.CcError) as e:
                 cga.solve_neighborhood(g, ls, **kw)
             assert e.value.code == cga.EINVAL
+
+
+def test_c3_with_local_search_matches_oracle(cga, orc, cpu_threads):
+    """NeighborhoodWithManyLocalSearchesCuda at its workload size (ER n=4096, p=0.1, graph-seed 3, batches of 256):
+    the best f, its labels (ties to the smaller start vertex) and the moves summed over all 4096 descents equal the
+    oracle's sequential enumeration, for every descent kernel and both G rebuilds. The reference's own check
+    (cuda/cc_verify_gpu.cu:37-54, 102-112) stops at n = 257."""
+    import oracle
+
+    g = cga.erdos_renyi(4096, 0.1, 3)
+    want = orc.neighborhood(oracle.Graph(g.bits, g.n, g.edges), True, threads=cpu_threads)
+    legs = [("auto", "popcount"), ("global", "popcount"), ("shared", "popcount"), ("wide", "popcount"),
+            ("auto", "mfma")]
+    with cga.DeviceGraph(g) as dg:
+        for kernel, rebuild in legs:
+            got = dg.solve_neighborhood(True, k=2, population=256, ls_kernel=kernel, rebuild=rebuild)
+            ctx = f"C3 kernel={kernel} rebuild={rebuild}"
+            assert got["objective"] == want["objective"], ctx
+            assert (got["labels"] == want["labels"]).all(), ctx + " labels"
+            assert got["accepted_moves"] == want["accepted_moves"], ctx + " moves"
+            assert got["local_searches"] == 4096, ctx

@@ -124,6 +124,29 @@ def test_wide_path_full_size_trajectory(cga, orc, cpu_threads, wide_graph, k, ls
     assert got["descent_name"] == expected
 
 
+def test_c5_graph_trajectory(cga, orc, c5):
+    """The wide-path workload's own graph (Chung-Lu n=100000, degree exponent 2.5, mean degree 64, graph-seed 4)
+    at k = 3 through the default wide plan (the chunk-summary kernel): pop 4 and 2 iterations follow the oracle's
+    trajectory bit for bit (labels, record trace, f, moves)."""
+    got = cga.solve_pbils(c5.graph, **c5.KW)
+    check_same(got, c5.want(orc), "C5 graph pop 4")
+    assert got["descent_name"] == "k3-chunk"
+
+
+def test_c5_full_config_properties(cga, orc, c5):
+    """The wide-path workload at full size (pop 128, 20 iterations, early stop off): the reported f is the
+    oracle's bitwise recount of the labels, the labels are a local optimum (the oracle's steepest-move scan finds
+    no improving move), the record never increases and ends at f, and the counters are those of 20 full
+    iterations."""
+    kw = dict(k=3, population=128, iterations=20, early_stop=20, gww=0.1, seed=42)
+    got = cga.solve_pbils(c5.graph, **kw)
+    og = c5.oracle_graph()
+    assert got["iterations_done"] == 20 and got["local_searches"] == 20 * 128
+    assert got["objective"] == orc.objective_bitwise(og, got["labels"])
+    assert orc.best_move(og, 3, got["labels"]) == (0, -1, -1)
+    assert (np.diff(got["trace"]) <= 0).all() and got["trace"][-1] == got["objective"]
+
+
 def local_optimum_violations(graph, labels, k):
     """Number of vertices with an improving single move, from G recomputed with numpy (no oracle code)."""
     adj = np.unpackbits(graph.bits.view(np.uint8), axis=1, bitorder="little")[:, : graph.n].astype(np.float32)

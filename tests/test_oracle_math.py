@@ -116,6 +116,43 @@ def test_local_search_reaches_local_optimum_and_counts_moves(orc):
                         assert delta >= 0
 
 
+def brute_best_move(adj, labels, k):
+    """The steepest move by enumeration in numpy: the smallest delta < 0, then the smallest v, then the smallest
+    target (src/cc_pbils_cpu.cpp:99-118)."""
+    n = len(labels)
+    z = np.zeros((n, k), dtype=np.int64)
+    z[np.arange(n), labels] = 1
+    g = adj.astype(np.int64) @ z
+    sizes = z.sum(axis=0)
+    own = g[np.arange(n), labels]
+    delta = (sizes[None, :] - sizes[labels][:, None] + 1) - 2 * (g - own[:, None])
+    delta[np.arange(n), labels] = 1
+    best = int(delta.min())
+    if best >= 0:
+        return 0, -1, -1
+    v, to = np.argwhere(delta == best)[0]  # row-major: smallest v, then smallest target
+    return best, int(v), int(to)
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 513, 700])
+def test_blocked_best_move_scan_equals_plain_scan(orc, n):
+    """The oracle's vectorisable two-step scan returns exactly the move of the reference's single in-order scan,
+    ties included: random labels (many improving moves), sparse graphs with few distinct deltas (many ties, also
+    across the 256-vertex blocks), and local optima (no move)."""
+    rng = np.random.default_rng(n)
+    for density in (0.0, 0.02, 0.5):
+        g = orc.erdos_renyi(n, density, int(rng.integers(1 << 30)), threads=1)
+        adj = g.dense()
+        for k in (1, 2, 3, 5, 9):
+            for trial in range(3):
+                labels = rng.integers(0, k, n).astype(np.int32)
+                if trial == 2:
+                    _, labels, _ = orc.local_search(g, k, labels)
+                want = brute_best_move(adj, labels, k)
+                assert orc.best_move(g, k, labels, plain=True) == want
+                assert orc.best_move(g, k, labels) == want
+
+
 def test_known_optima(orc):
     # complete graphs reach 0 with one cluster (src/cc_verify.cpp:131-143)
     for n in (2, 6, 33):
