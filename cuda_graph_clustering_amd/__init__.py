@@ -141,6 +141,8 @@ ABI_SYMBOLS = [
     "cc_objective_direct",
     "cc_objective_bitwise",
     "cc_graph_load_tags",
+    "cc_graph_load",
+    "cc_graph_save",
     "cc_host_free",
     "cc_sample_without_replacement",
     "cc_hip_device_info",
@@ -200,6 +202,13 @@ def _load():
                                        ctypes.c_uint64, ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_uint32),
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_graph_load_tags.restype = ctypes.c_int
+    lib.cc_graph_load.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(_u64p),
+                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p,
+                                  ctypes.c_size_t]
+    lib.cc_graph_load.restype = ctypes.c_int
+    lib.cc_graph_save.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p,
+                                  ctypes.c_size_t]
+    lib.cc_graph_save.restype = ctypes.c_int
     lib.cc_host_free.argtypes = [ctypes.c_void_p]
     lib.cc_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                                   ctypes.POINTER(ctypes.c_uint32)]
@@ -273,6 +282,35 @@ def load_tags(path, similarity="jaccard", threshold=0.5, n=0, seed=1):
     finally:
         lib.cc_host_free(buf)
     return Graph(bits, size.value, edges.value)
+
+
+GRAPH_FORMATS = {"matrix": 0, "json": 1, "edges": 2}
+
+
+def _take_bits(buf, n):
+    try:
+        count = n * words64(n)
+        return np.ctypeslib.as_array(buf, shape=(count,)).copy() if count else np.zeros(0, dtype=np.uint64)
+    finally:
+        lib.cc_host_free(buf)
+
+
+def load_graph(path, fmt="matrix"):
+    """An instance file of the command line: "matrix" (Graph::LoadMatrix), "json" (the "graph" block of a baseline
+    result, Graph::LoadBaselineJson) or "edges" (SNAP/CSV edge list, Graph::LoadEdgeList)."""
+    size = ctypes.c_uint32(0)
+    edges = ctypes.c_uint64(0)
+    buf = _u64p()
+    err = ctypes.create_string_buffer(512)
+    _check(lib.cc_graph_load(os.fsencode(path), GRAPH_FORMATS[fmt], ctypes.byref(buf), ctypes.byref(size),
+                             ctypes.byref(edges), err, 512), err)
+    return Graph(_take_bits(buf, size.value), size.value, edges.value)
+
+
+def save_graph(graph, path, fmt="matrix"):
+    """Writes a graph as "matrix" (Graph::SaveMatrix), "json" (Graph::SaveBaselineJson) or "edges"."""
+    err = ctypes.create_string_buffer(512)
+    _check(lib.cc_graph_save(graph.ptr(), graph.n, os.fsencode(path), GRAPH_FORMATS[fmt], err, 512), err)
 
 
 def sample_without_replacement(total, n, seed):

@@ -20,6 +20,7 @@
 
 #include "cc_gpu.hpp"
 #include "cc_graph.hpp"
+#include "cc_graph_io.hpp"
 #include "cc_hip.h"
 #include "cc_math.hpp"
 #include "cc_pbils.hpp"
@@ -72,14 +73,6 @@ uint64_t CountEdges(const Word* bits, size_t count) {
   uint64_t ones = 0;
   for (size_t i = 0; i < count; ++i) ones += static_cast<uint64_t>(__builtin_popcountll(bits[i]));
   return ones / 2;
-}
-
-std::string ReadWholeFile(const std::string& path) {
-  std::ifstream in(path, std::ios::binary);
-  if (!in) throw std::runtime_error("cannot open file: " + path);
-  std::stringstream buffer;
-  buffer << in.rdbuf();
-  return buffer.str();
 }
 
 void GeneratorErdosRenyi(unsigned n, double p, uint64_t seed, Word* bits, int threads) {
@@ -163,183 +156,9 @@ Graph Graph::ChungLu(unsigned n, double gamma, double mean_degree, uint64_t seed
   return g;
 }
 
-Graph Graph::LoadMatrix(const std::string& path) {
-  std::ifstream in(path);
-  if (!in) throw std::runtime_error("cannot open graph file: " + path);
-  unsigned n = 0;
-  in >> n;
-  if (!in || n == 0) throw std::runtime_error("bad graph header in " + path);
-  Graph g(n);
-  for (unsigned i = 0; i < n; ++i) {
-    for (unsigned j = 0; j < n; ++j) {
-      int ch = in.get();
-      while (ch != EOF && std::isspace(ch)) ch = in.get();
-      if (ch == EOF) throw std::runtime_error("truncated matrix in " + path);
-      if (ch == '1' && j > i) g.AddEdge(i, j);
-    }
-  }
-  return g;
-}
-
-void Graph::SaveMatrix(const std::string& path) const {
-  std::ofstream out(path);
-  if (!out) throw std::runtime_error("cannot write graph file: " + path);
-  out << n_ << "\n";
-  std::string line(n_, '0');
-  for (unsigned i = 0; i < n_; ++i) {
-    for (unsigned j = 0; j < n_; ++j) line[j] = IsJoined(i, j) ? '1' : '0';
-    out << line << "\n";
-  }
-}
-
-Graph Graph::LoadBaselineJson(const std::string& path) {
-  const std::string text = ReadWholeFile(path);
-  const size_t key = text.find("\"graph\"");
-  if (key == std::string::npos) throw std::runtime_error("no \"graph\" key in " + path);
-  size_t pos = text.find('[', key);
-  if (pos == std::string::npos) throw std::runtime_error("malformed \"graph\" block in " + path);
-  std::vector<std::vector<char>> rows;
-  std::vector<char> current;
-  int depth = 0;
-  for (; pos < text.size(); ++pos) {
-    const char ch = text[pos];
-    if (ch == '[') {
-      if (++depth == 2) current.clear();
-    } else if (ch == ']') {
-      if (depth == 2) rows.push_back(current);
-      if (--depth == 0) break;
-    } else if (depth == 2 && (ch == '0' || ch == '1')) {
-      current.push_back(static_cast<char>(ch - '0'));
-    }
-  }
-  if (rows.empty()) throw std::runtime_error("empty \"graph\" block in " + path);
-  const unsigned n = static_cast<unsigned>(rows.size());
-  Graph g(n);
-  for (unsigned i = 0; i < n; ++i) {
-    if (rows[i].size() != n) {
-      throw std::runtime_error("row " + std::to_string(i) + " of \"graph\" is not n long in " + path);
-    }
-    for (unsigned j = i + 1; j < n; ++j) {
-      if (rows[i][j]) g.AddEdge(i, j);
-    }
-  }
-  return g;
-}
-
-void Graph::SaveBaselineJson(const std::string& path) const {
-  std::ofstream out(path);
-  if (!out) throw std::runtime_error("cannot write json file: " + path);
-  out << "{\n\"size\": " << n_ << ",\n\"density\": " << Density() << ",\n\"graph\": [\n";
-  for (unsigned i = 0; i < n_; ++i) {
-    out << "[";
-    for (unsigned j = 0; j < n_; ++j) out << (IsJoined(i, j) ? 1 : 0) << (j + 1 != n_ ? "," : "");
-    out << (i + 1 != n_ ? "],\n" : "]\n");
-  }
-  out << "]\n}\n";
-}
-
-Graph Graph::LoadEdgeList(const std::string& path) {
-  const std::string text = ReadWholeFile(path);
-  std::vector<uint64_t> ends;
-  size_t pos = 0;
-  auto read_id = [&](uint64_t& id) {
-    while (pos < text.size() && (text[pos] == ' ' || text[pos] == '\t' || text[pos] == ',')) ++pos;
-    if (pos >= text.size() || !std::isdigit(static_cast<unsigned char>(text[pos]))) return false;
-    for (id = 0; pos < text.size() && std::isdigit(static_cast<unsigned char>(text[pos])); ++pos) {
-      if (id > (UINT64_MAX - 9) / 10) throw std::runtime_error("vertex id does not fit 64 bits in " + path);
-      id = id * 10 + static_cast<uint64_t>(text[pos] - '0');
-    }
-    return true;
-  };
-  while (pos < text.size()) {
-    const size_t line_end = std::min(text.find('\n', pos), text.size());
-    uint64_t u = 0, v = 0;
-    if (text[pos] != '#' && text[pos] != '%' && read_id(u) && read_id(v)) {
-      ends.push_back(u);
-      ends.push_back(v);
-    }
-    pos = line_end + 1;
-  }
-  if (ends.empty()) throw std::runtime_error("no edges in " + path);
-  std::vector<uint64_t> ids(ends);
-  std::sort(ids.begin(), ids.end());
-  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
-  if (ids.size() > UINT32_MAX) throw std::runtime_error("too many vertices in " + path);
-  auto rank = [&](uint64_t id) {
-    return static_cast<unsigned>(std::lower_bound(ids.begin(), ids.end(), id) - ids.begin());
-  };
-  Graph g(static_cast<unsigned>(ids.size()));
-  for (size_t i = 0; i < ends.size(); i += 2) g.AddEdge(rank(ends[i]), rank(ends[i + 1]));
-  return g;
-}
-
-void Graph::SaveEdgeList(const std::string& path) const {
-  std::ofstream out(path);
-  if (!out) throw std::runtime_error("cannot write edge list: " + path);
-  out << "# " << n_ << " vertices, " << edges_ << " edges\n";
-  for (unsigned i = 0; i < n_; ++i) {
-    for (unsigned j = i + 1; j < n_; ++j) {
-      if (IsJoined(i, j)) out << i << ' ' << j << '\n';
-    }
-  }
-}
-
 // ---- tag-similarity graph ----------------------------------------------------------------------------------------
 
 namespace {
-
-// {"key": ["tag", ...], ...} -> per object, in file order, its tag ids sorted and without repeats. Only this shape
-// is read (the dataset files have exactly it); inside a string a backslash takes the next character as it is, which
-// is enough to tell tags apart. Keys are not used: an object's vertex is its position in the file.
-std::vector<std::vector<int>> ParseTagSets(const std::string& path) {
-  const std::string text = ReadWholeFile(path);
-  size_t pos = 0;
-  auto fail = [&](const char* what) -> void {
-    throw std::runtime_error(std::string("malformed tags json (") + what + ") at byte " + std::to_string(pos) +
-                             " of " + path);
-  };
-  auto peek = [&]() {
-    while (pos < text.size() && std::isspace(static_cast<unsigned char>(text[pos]))) ++pos;
-    return pos < text.size() ? text[pos] : '\0';
-  };
-  auto take = [&](char ch) {
-    if (peek() != ch) fail("unexpected character");
-    ++pos;
-  };
-  std::string token;
-  auto string = [&]() {
-    take('"');
-    token.clear();
-    for (; pos < text.size() && text[pos] != '"'; ++pos) {
-      if (text[pos] == '\\' && ++pos >= text.size()) break;
-      token += text[pos];
-    }
-    if (pos >= text.size()) fail("unterminated string");
-    ++pos;
-  };
-  std::unordered_map<std::string, int> ids;
-  std::vector<std::vector<int>> sets;
-  take('{');
-  if (peek() == '}') return sets;
-  while (true) {
-    string();
-    take(':');
-    take('[');
-    std::vector<int> tags;
-    while (peek() != ']') {
-      if (!tags.empty()) take(',');
-      string();
-      tags.push_back(ids.emplace(token, static_cast<int>(ids.size())).first->second);
-    }
-    take(']');
-    std::sort(tags.begin(), tags.end());
-    tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
-    sets.push_back(std::move(tags));
-    if (peek() == '}') break;
-    take(',');
-  }
-  return sets;
-}
 
 enum class TagMeasure { kJaccard, kCosine, kDice, kOverlap };
 
@@ -385,7 +204,7 @@ std::vector<unsigned> SampleWithoutReplacement(unsigned total, unsigned n, uint6
 Graph Graph::LoadTags(const std::string& path, const std::string& similarity, double threshold, unsigned n,
                       uint64_t seed) {
   const TagMeasure measure = ParseTagMeasure(similarity);
-  const std::vector<std::vector<int>> sets = ParseTagSets(path);
+  const std::vector<std::vector<int>> sets = ReadTagFile(path);
   if (sets.empty()) throw std::runtime_error("no objects in " + path);
   const std::vector<unsigned> objects = SampleWithoutReplacement(static_cast<unsigned>(sets.size()), n, seed);
   const unsigned size = static_cast<unsigned>(objects.size());
@@ -471,55 +290,111 @@ Graph Graph::LoadTags(const std::string& path, const std::string& similarity, do
 
 // ---- recounts and GWW helpers -----------------------------------------------------------------------------------
 
-long long ObjectiveDirect(const Graph& graph, const std::vector<int>& labels) {
-  const unsigned n = graph.Size();
-  if (labels.size() != n) throw std::runtime_error("labels length != graph size");
-  long long distance = 0;
-  for (unsigned i = 0; i < n; ++i) {
-    for (unsigned j = i + 1; j < n; ++j) {
-      if ((labels[i] == labels[j]) != graph.IsJoined(i, j)) ++distance;
-    }
+namespace {
+
+void CheckLabels(const Graph& graph, const std::vector<int>& labels) {
+  if (labels.size() != graph.Size()) {
+    throw std::runtime_error("a clustering of " + std::to_string(labels.size()) + " labels for a graph of " +
+                             std::to_string(graph.Size()) + " vertices");
   }
-  return distance;
+  for (const int label : labels) {
+    if (label < 0) throw std::runtime_error("labels must be non-negative");
+  }
 }
 
-long long ObjectiveBitwise(const Graph& graph, const std::vector<int>& labels) {
-  const unsigned n = graph.Size();
-  if (labels.size() != n) throw std::runtime_error("labels length != graph size");
-  const unsigned words = graph.WordsPerRow();
-  int k = 0;
-  for (const int label : labels) {
-    if (label < 0) throw std::runtime_error("negative label");
-    k = std::max(k, label + 1);
-  }
-  std::vector<Word> masks(static_cast<size_t>(std::max(k, 1)) * words, 0);
-  for (unsigned v = 0; v < n; ++v) masks[static_cast<size_t>(labels[v]) * words + v / kWordBits] |= Word(1) << (v % kWordBits);
-  constexpr unsigned kRowsPerItem = 256;
-  const int items = static_cast<int>((n + kRowsPerItem - 1) / kRowsPerItem);
-  std::vector<long long> sums(static_cast<size_t>(std::max(items, 1)), 0);
-  ParallelFor(items, 0, [&](int item) {
-    long long acc = 0;
-    for (unsigned v = item * kRowsPerItem; v < std::min(n, (item + 1) * kRowsPerItem); ++v) {
-      const Word* row = graph.Row(v);
-      const Word* mask = masks.data() + static_cast<size_t>(labels[v]) * words;
-      for (unsigned w = 0; w < words; ++w) acc += __builtin_popcountll(row[w] ^ mask[w]);
-      acc -= 1;  // the vertex's own bit: set in its mask, clear in its row
+// Rows [0, n) split into one contiguous range per thread; body(first, last, sums) adds the range's counts to sums[2].
+template <typename Body>
+void SumOverRows(unsigned n, long long* out, Body&& body) {
+  const int threads = std::max(1, std::min(HardwareThreads(), static_cast<int>(n / 512) + 1));
+  std::vector<long long> part(2 * static_cast<size_t>(threads), 0);
+  std::vector<std::thread> pool;
+  const unsigned step = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    const unsigned first = std::min<unsigned>(n, t * step), last = std::min<unsigned>(n, first + step);
+    long long* sums = part.data() + 2 * t;
+    if (t + 1 == threads) {
+      body(first, last, sums);
+    } else {
+      pool.emplace_back([&body, first, last, sums] { body(first, last, sums); });
     }
-    sums[item] = acc;
+  }
+  for (auto& th : pool) th.join();
+  out[0] = out[1] = 0;
+  for (int t = 0; t < threads; ++t) {
+    out[0] += part[2 * t];
+    out[1] += part[2 * t + 1];
+  }
+}
+
+}  // namespace
+
+// The definition, pair by pair: a pair disagrees when "same cluster" and "joined" differ. For the 64 partners j of a
+// word, "same cluster" is collected into a mask and compared with the row word at once; only j > i is counted.
+long long ObjectiveDirect(const Graph& graph, const std::vector<int>& labels) {
+  CheckLabels(graph, labels);
+  const unsigned n = graph.Size();
+  long long disagreements = 0;
+  for (unsigned i = 0; i + 1 < n; ++i) {
+    const Word* row = graph.Row(i);
+    for (unsigned w = (i + 1) / kWordBits; w * kWordBits < n; ++w) {
+      Word same = 0;
+      const unsigned base = w * kWordBits;
+      for (unsigned b = 0; b < kWordBits && base + b < n; ++b) same |= Word(labels[base + b] == labels[i]) << b;
+      Word counted = base + kWordBits <= n ? ~Word(0) : (Word(1) << (n - base)) - 1;
+      if (base <= i) counted &= ~Word(0) << (i + 1 - base);  // partners above i only
+      disagreements += __builtin_popcountll((same ^ row[w]) & counted);
+    }
+  }
+  return disagreements;
+}
+
+// f = (m - W) + (sum_c C(n_c, 2) - W): the edges between clusters plus the missing edges inside clusters, with the
+// intra-cluster edges W and the edges m counted afresh from the bit rows (popcount of row & own-cluster mask, and of
+// the row) on all cores; nothing of the solver's G or f is used.
+long long ObjectiveBitwise(const Graph& graph, const std::vector<int>& labels) {
+  CheckLabels(graph, labels);
+  const unsigned n = graph.Size();
+  const unsigned words = graph.WordsPerRow();
+  const int k = labels.empty() ? 1 : *std::max_element(labels.begin(), labels.end()) + 1;
+  std::vector<Word> member(static_cast<size_t>(k) * words, Word(0));
+  std::vector<long long> size(static_cast<size_t>(k), 0);
+  for (unsigned v = 0; v < n; ++v) {
+    member[static_cast<size_t>(labels[v]) * words + v / kWordBits] |= Word(1) << (v % kWordBits);
+    ++size[labels[v]];
+  }
+  // counts[0]: sum of degrees (2m), counts[1]: sum over v of v's neighbours in its own cluster (2W)
+  long long counts[2];
+  SumOverRows(n, counts, [&](unsigned first, unsigned last, long long* sums) {
+    long long degree = 0, inside = 0;
+    for (unsigned v = first; v < last; ++v) {
+      const Word* row = graph.Row(v);
+      const Word* own = member.data() + static_cast<size_t>(labels[v]) * words;
+      for (unsigned w = 0; w < words; ++w) {
+        degree += __builtin_popcountll(row[w]);
+        inside += __builtin_popcountll(row[w] & own[w]);
+      }
+    }
+    sums[0] = degree;
+    sums[1] = inside;
   });
-  long long total = 0;
-  for (const long long s : sums) total += s;
-  return total / 2;
+  long long pairs_inside = 0;
+  for (const long long c : size) pairs_inside += c * (c - 1) / 2;
+  const long long m = counts[0] / 2, w_inside = counts[1] / 2;
+  return (m - w_inside) + (pairs_inside - w_inside);
 }
 
 int CountClustersUsed(const std::vector<int>& labels, int k) {
-  std::vector<char> seen(static_cast<size_t>(std::max(k, 1)), 0);
+  if (k <= 0) return 0;
+  std::vector<bool> seen(static_cast<size_t>(k), false);
   for (const int label : labels) {
-    if (label >= 0 && label < k) seen[label] = 1;
+    if (label >= 0 && label < k) seen[static_cast<size_t>(label)] = true;
   }
-  int used = 0;
-  for (const char s : seen) used += s;
-  return used;
+  return static_cast<int>(std::count(seen.begin(), seen.end(), true));
+}
+
+int ResolveThreads(const PbilsParams& params) {
+  const int threads = params.threads >= 1 ? params.threads : HardwareThreads();
+  return params.population >= 1 ? std::min(threads, params.population) : threads;
 }
 
 int WinnerCopies(double gww, int population) { return static_cast<int>(gww * population); }
@@ -531,10 +406,13 @@ std::vector<int> RankSlots(const long long* f, int population) {
   return order;
 }
 
+// The parameter rules of both backends (reference src/cc_pbils_cpu.cpp:177-181).
 void CheckParams(const PbilsParams& params) {
-  if (params.k < 1) throw std::runtime_error("k must be >= 1");
-  if (params.population < 1) throw std::runtime_error("population must be >= 1");
-  if (params.gww < 0.0 || params.gww > kMaxGwwShare) throw std::runtime_error("gww must be in [0, 0.5]");
+  if (params.k < 1) throw std::runtime_error("the cluster bound k must be at least 1");
+  if (params.population < 1) throw std::runtime_error("the population must hold at least one individual");
+  if (!(params.gww >= 0.0 && params.gww <= kMaxGwwShare)) {
+    throw std::runtime_error("the GWW share must lie in [0, 0.5]");
+  }
 }
 
 // ---- GPU entry points over the C ABI ----------------------------------------------------------------------------
@@ -638,6 +516,27 @@ PbilsResult SolveNeighborhoodWithManyLocalSearchesGpu(const Graph& graph, const 
 
 }  // namespace cc
 
+namespace cc {
+namespace {
+
+// A host copy of a graph's bits for the caller (released with cc_host_free).
+uint64_t* HandOut(const Graph& g) {
+  const std::vector<Word>& bits = g.Bits();
+  uint64_t* out = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, bits.size()) * sizeof(uint64_t)));
+  if (!out) throw std::runtime_error("out of host memory for the bitset");
+  std::copy(bits.begin(), bits.end(), out);
+  return out;
+}
+
+int HostError(const std::exception& e, char* err, size_t err_len) {
+  if (err && err_len) std::snprintf(err, err_len, "%s", e.what());
+  return CC_HIP_EINVAL;
+}
+
+}  // namespace
+
+}  // namespace cc
+
 // ---- the generator / recount part of the C ABI ----------------------------------------------------------------
 
 extern "C" {
@@ -666,11 +565,7 @@ int cc_graph_load_tags(const char* path, const char* similarity, double threshol
   *bits_out = nullptr;
   try {
     const cc::Graph g = cc::Graph::LoadTags(path, similarity, threshold, n, seed);
-    const std::vector<cc::Word>& bits = g.Bits();
-    uint64_t* out = static_cast<uint64_t*>(std::malloc(std::max<size_t>(1, bits.size()) * sizeof(uint64_t)));
-    if (!out) throw std::runtime_error("out of host memory for the bitset");
-    std::copy(bits.begin(), bits.end(), out);
-    *bits_out = out;
+    *bits_out = cc::HandOut(g);
     *n_out = g.Size();
     *edges_out = g.EdgeCount();
     return CC_HIP_OK;
@@ -681,6 +576,41 @@ int cc_graph_load_tags(const char* path, const char* similarity, double threshol
 }
 
 void cc_host_free(void* p) { std::free(p); }
+
+int cc_graph_load(const char* path, int32_t format, uint64_t** bits_out, uint32_t* n_out, uint64_t* edges_out,
+                  char* err, size_t err_len) {
+  *bits_out = nullptr;
+  try {
+    cc::Graph g;
+    switch (format) {
+      case CC_GRAPH_MATRIX: g = cc::Graph::LoadMatrix(path); break;
+      case CC_GRAPH_BASELINE_JSON: g = cc::Graph::LoadBaselineJson(path); break;
+      case CC_GRAPH_EDGE_LIST: g = cc::Graph::LoadEdgeList(path); break;
+      default: throw std::runtime_error("unknown graph file format " + std::to_string(format));
+    }
+    *bits_out = cc::HandOut(g);
+    *n_out = g.Size();
+    *edges_out = g.EdgeCount();
+    return CC_HIP_OK;
+  } catch (const std::exception& e) {
+    return cc::HostError(e, err, err_len);
+  }
+}
+
+int cc_graph_save(const uint64_t* bits, uint32_t n, const char* path, int32_t format, char* err, size_t err_len) {
+  try {
+    const cc::Graph g = cc::Graph::FromBits(std::vector<cc::Word>(bits, bits + static_cast<size_t>(n) * cc::WordsFor(n)), n);
+    switch (format) {
+      case CC_GRAPH_MATRIX: g.SaveMatrix(path); break;
+      case CC_GRAPH_BASELINE_JSON: g.SaveBaselineJson(path); break;
+      case CC_GRAPH_EDGE_LIST: g.SaveEdgeList(path); break;
+      default: throw std::runtime_error("unknown graph file format " + std::to_string(format));
+    }
+    return CC_HIP_OK;
+  } catch (const std::exception& e) {
+    return cc::HostError(e, err, err_len);
+  }
+}
 
 void cc_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
   const std::vector<unsigned> order = cc::SampleWithoutReplacement(total, n, seed);

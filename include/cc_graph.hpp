@@ -58,6 +58,10 @@ class Graph {
                         uint64_t seed);
 
  private:
+  // Sets bit (j, i) for every bit (i, j) above the diagonal and counts those as the edges: the loaders fill the upper
+  // triangle only.
+  void MirrorUpperTriangle();
+
   unsigned n_ = 0;
   unsigned words_per_row_ = 1;
   uint64_t edges_ = 0;

@@ -215,6 +215,20 @@ uint64_t cc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_
 int cc_graph_load_tags(const char* path, const char* similarity, double threshold, uint32_t n, uint64_t seed,
                        uint64_t** bits_out, uint32_t* n_out, uint64_t* edges_out, char* err, size_t err_len);
 void cc_host_free(void* p);
+/* Instance files of the reference's command line (include/cc_graph.hpp:35-48, src/cc_graph.cpp:179-306): the 0/1
+ * matrix text (n, then n rows of n cells '0'/'1', spaced or not), the "graph" block of a baseline result JSON, and a
+ * SNAP/CSV edge list (two ids per line, a vertex is the rank of its id; loading only). On success *bits_out is a host
+ * buffer of *n_out * cc_words64(*n_out) words that the caller releases with cc_host_free. CC_HIP_EINVAL for an
+ * unreadable or malformed file. */
+#define CC_GRAPH_MATRIX 0
+#define CC_GRAPH_BASELINE_JSON 1
+#define CC_GRAPH_EDGE_LIST 2
+int cc_graph_load(const char* path, int32_t format, uint64_t** bits_out, uint32_t* n_out, uint64_t* edges_out,
+                  char* err, size_t err_len);
+/* Writes bits (n rows of cc_words64(n) words, symmetric, zero diagonal) as CC_GRAPH_MATRIX, CC_GRAPH_BASELINE_JSON
+ * (byte for byte the reference's Graph::SaveMatrix / SaveBaselineJson) or CC_GRAPH_EDGE_LIST ("# n vertices, m
+ * edges", then "i j" with i < j per line). */
+int cc_graph_save(const uint64_t* bits, uint32_t n, const char* path, int32_t format, char* err, size_t err_len);
 /* The reference's SampleWithoutReplacement (include/cc_graph.hpp:62): out receives the first min(n, total) entries
  * of a partial Fisher-Yates shuffle of 0 .. total - 1 on StreamSeed(seed, 0x7A65, 0), or 0 .. total - 1 when n = 0
  * or n >= total. */

@@ -1,12 +1,15 @@
 // cc_pbils.hpp — PBILS parameters and result, and the host helpers shared by the GPU front end: the record
 // bookkeeping rules, GWW ranking and the independent objective recounts that verify a result.
 //
-// Same names, fields and defaults as the reference (include/cc_pbils.hpp:26-137 of the reference). The CPU solver
-// of the reference (State, LocalSearch, SolveCpu) has no counterpart in this library: the CPU path lives in the
-// test oracle only.
+// Same names, fields and defaults as the reference (include/cc_pbils.hpp:26-137 of the reference), so that the
+// reference's own callers (its command line, include/cc_cli.hpp, and cuda/main_gpu.cu) build against this header
+// unchanged. The CPU solver of the reference (State, LocalSearch, SolveCpu) has no counterpart in this library: the
+// CPU path lives in the test oracle only.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "cc_graph.hpp"
@@ -52,12 +55,50 @@ struct PbilsResult {
   double rebuild_ms = 0.0;
 };
 
+// The solver clock, the stall count and the stop rule of a PBILS run, as both of the reference's backends keep them
+// (include/cc_pbils.hpp:62-85 of the reference). The GPU runtime applies the same rule on the device (the record
+// kernel); this class is the host-side form for callers that drive iterations themselves, and the clock starts at
+// construction.
+class Progress {
+ public:
+  explicit Progress(const PbilsParams& params) : params_(params), start_(std::chrono::steady_clock::now()) {}
+
+  double Seconds() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
+  }
+
+  // Books one finished iteration (improved: it produced a new record). Returns true when the run must stop: early_stop
+  // iterations in a row without a record, or the time limit (when positive) reached.
+  bool Next(PbilsResult& result, bool improved) {
+    result.iterations_done += 1;
+    result.local_searches += params_.population;
+    stall_ = improved ? 0 : stall_ + 1;
+    if (params_.verbose) {
+      std::printf("  iter %3d  record %lld  stall %d  %.3fs\n", result.iterations_done, result.objective, stall_,
+                  Seconds());
+    }
+    const bool stalled = stall_ >= params_.early_stop;
+    const bool out_of_time = params_.time_limit_sec > 0.0 && Seconds() >= params_.time_limit_sec;
+    return stalled || out_of_time;
+  }
+
+ private:
+  const PbilsParams& params_;
+  std::chrono::steady_clock::time_point start_;
+  int stall_ = 0;
+};
+
 // f by the definition, pairwise, O(n^2) (reference src/cc_pbils_cpu.cpp:12-23).
 long long ObjectiveDirect(const Graph& graph, const std::vector<int>& labels);
 // f over bit rows: popcount(row XOR own-cluster mask) - 1 per vertex, halved; O(n^2/64) on all cores
 // (reference src/cc_pbils_cpu.cpp:25-53).
 long long ObjectiveBitwise(const Graph& graph, const std::vector<int>& labels);
 int CountClustersUsed(const std::vector<int>& labels, int k);
+
+// Worker threads of a CPU solve: params.threads, or every hardware thread when it is <= 0, and never more than one per
+// individual (reference include/cc_pbils.hpp:119, src/cc_pbils_cpu.cpp:163-166). The GPU path does not use it; the
+// reference's command line prints it for its CPU backend.
+int ResolveThreads(const PbilsParams& params);
 
 constexpr double kMaxGwwShare = 0.5;
 // The worst floor(gww * population) slots receive copies of the best (reference src/cc_pbils_cpu.cpp:168).

@@ -296,6 +296,11 @@ class Reference:
                                       ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.c_char_p, ctypes.c_size_t]
         lib.ref_load_tags.restype = ctypes.c_int64
+        lib.ref_load_graph.argtypes = [ctypes.c_char_p, ctypes.c_int32, _u64p, ctypes.c_uint64,
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_size_t]
+        lib.ref_load_graph.restype = ctypes.c_int64
+        lib.ref_save_graph.argtypes = [_u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int32]
+        lib.ref_save_graph.restype = ctypes.c_int
         lib.ref_sample_without_replacement.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                                        ctypes.POINTER(ctypes.c_uint32)]
 
@@ -363,6 +368,27 @@ class Reference:
         bits = np.zeros((size, words_per_row(size)), dtype=np.uint64)
         self.lib.ref_load_tags(*args, _ptr(bits, _u64p), bits.size, ctypes.byref(edges), err, 512)
         return Graph(bits, size, edges.value)
+
+    GRAPH_FORMATS = {"matrix": 0, "json": 1, "edges": 2}
+
+    def load_graph(self, path, fmt):
+        """Graph::LoadMatrix / LoadBaselineJson / LoadEdgeList of the reference; raises RuntimeError with its
+        message."""
+        edges = ctypes.c_uint64(0)
+        err = ctypes.create_string_buffer(512)
+        code = self.GRAPH_FORMATS[fmt]
+        size = self.lib.ref_load_graph(os.fsencode(path), code, None, 0, ctypes.byref(edges), err, 512)
+        if size < 0:
+            raise RuntimeError(err.value.decode())
+        bits = np.zeros((size, words_per_row(size)), dtype=np.uint64)
+        self.lib.ref_load_graph(os.fsencode(path), code, _ptr(bits, _u64p), bits.size, ctypes.byref(edges), err, 512)
+        return Graph(bits, size, edges.value)
+
+    def save_graph(self, g, path, fmt):
+        """Graph::SaveMatrix / SaveBaselineJson of the reference."""
+        bits = np.ascontiguousarray(g.bits, dtype=np.uint64)
+        if self.lib.ref_save_graph(_ptr(bits, _u64p), g.n, g.words, os.fsencode(path), self.GRAPH_FORMATS[fmt]):
+            raise RuntimeError("reference save failed")
 
     def sample_without_replacement(self, total, n, seed):
         out = np.zeros(max(total, 1), dtype=np.uint32)

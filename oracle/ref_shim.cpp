@@ -164,6 +164,40 @@ int64_t ref_load_tags(const char* path, const char* similarity, double threshold
   }
 }
 
+// The reference's instance files: format 0 Graph::LoadMatrix, 1 Graph::LoadBaselineJson, 2 Graph::LoadEdgeList.
+// Returns -1 with a message, or the vertex count; bits are written only if capacity_words holds them.
+int64_t ref_load_graph(const char* path, int32_t format, uint64_t* bits, uint64_t capacity_words, uint64_t* edges_out,
+                       char* err, size_t err_len) {
+  try {
+    const cc::Graph graph = format == 0   ? cc::Graph::LoadMatrix(path)
+                            : format == 1 ? cc::Graph::LoadBaselineJson(path)
+                                          : cc::Graph::LoadEdgeList(path);
+    const uint32_t size = graph.Size();
+    const uint32_t words = (size + 63) / 64 > 0 ? (size + 63) / 64 : 1;
+    *edges_out = graph.EdgeCount();
+    if ((uint64_t)size * words <= capacity_words) ToBits(graph, bits, words);
+    return size;
+  } catch (const std::exception& e) {
+    SetError(err, err_len, e.what());
+    return -1;
+  }
+}
+
+// Graph::SaveMatrix (format 0) or Graph::SaveBaselineJson (format 1) of the reference.
+int ref_save_graph(const uint64_t* bits, uint32_t n, uint32_t words, const char* path, int32_t format) {
+  try {
+    const cc::Graph graph = FromBits(bits, n, words);
+    if (format == 0) {
+      graph.SaveMatrix(path);
+    } else {
+      graph.SaveBaselineJson(path);
+    }
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 void ref_sample_without_replacement(uint32_t total, uint32_t n, uint64_t seed, uint32_t* out) {
   const std::vector<unsigned> order = cc::SampleWithoutReplacement(total, n, seed);
   for (size_t i = 0; i < order.size(); ++i) out[i] = order[i];

@@ -98,3 +98,23 @@ def test_tune_table_load_validates_entries(tmp_path):
             assert e.value.code == cga.EINVAL
     finally:
         cga.tune_clear()
+
+
+def test_reference_command_line_compiles_against_this_include():
+    """The reference's GPU command line (cuda/main_gpu.cu with its include/cc_cli.hpp) compiles unchanged against
+    include/ of this library: every declaration it uses (SolveGpu and the Neighborhood entry points, Graph and its
+    loaders, PbilsParams, PbilsResult, ResolveThreads, ObjectiveDirect, ObjectiveBitwise) is there with its
+    signature. Syntax check only; oracle/Makefile links the same source into oracle/_ref/cc_gpu_ref."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    reference = "/root/reference"
+    if not os.path.exists(os.path.join(reference, "cuda", "main_gpu.cu")):
+        pytest.skip("the reference sources are not on this machine")
+    with tempfile.TemporaryDirectory() as tmp:
+        os.symlink(os.path.join(reference, "include", "cc_cli.hpp"), os.path.join(tmp, "cc_cli.hpp"))
+        proc = subprocess.run([shutil.which("g++") or "g++", "-std=c++17", "-fsyntax-only", "-x", "c++", "-iquote", tmp,
+                               "-I", os.path.join(ROOT, "include"), os.path.join(reference, "cuda", "main_gpu.cu")],
+                              capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr

@@ -108,3 +108,46 @@ def test_tags_graph_through_cli(cga, orc, cpu_threads, tmp_path):
     assert (saved == dense).all()
     proc = run(["--tags", "tags.json", "--similarity", "euclid", *PBILS], tmp_path, expect_ok=False)
     assert proc.returncode == 1 and "error:" in proc.stdout
+
+
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "cc_gpu_ref")
+
+
+def without_times(stdout):
+    """stdout with the clock readings and the written paths masked: what must be identical between two runs."""
+    import re
+
+    return [re.sub(r"\d+\.\d+s", "T", line) for line in stdout.splitlines() if not line.startswith("written")]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CLI), reason="the reference's command line is built only where its sources are")
+def test_reference_command_line_runs_on_this_library(cga, orc, cpu_threads, tmp_path):
+    """The reference's own cc_gpu (cuda/main_gpu.cu with include/cc_cli.hpp, compiled unchanged against include/
+    and linked to libcc_hip.so) solves through this library: its result line and --out JSON match the oracle, its
+    --labels-out is the oracle's labels, and its stdout equals this package's cc_gpu line for line, times aside."""
+    import oracle
+
+    g = cga.erdos_renyi(300, 0.3, 5)
+    want = orc.solve_pbils(oracle.Graph(g.bits, g.n, g.edges), threads=cpu_threads, k=3, population=16,
+                           iterations=5, early_stop=5, seed=7, gww=0.1)
+    args = ["--n", 300, "--density", 0.3, "--graph-seed", 5, *PBILS, "--runs", 2]
+    proc = subprocess.run([REF_CLI] + [str(a) for a in args] + ["--out", "r.json", "--labels-out", "l.txt"],
+                          cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert objective_line(proc.stdout) == want["objective"]
+    result = json.loads((tmp_path / "r.json").read_text())
+    assert result["backend"] == "gpu" and result["runs"] == 2
+    assert result["objective function value"] <= want["objective"]  # the better of seeds 7 and 8
+    labels = np.array([int(x) for x in (tmp_path / "l.txt").read_text().split()])
+    assert result["clustering vector"] == labels.tolist()
+    assert cga.objective_direct(g, labels) == result["objective function value"]
+    ours = run(args, tmp_path).stdout
+    assert without_times(ours) == without_times(proc.stdout)
+    small = cga.erdos_renyi(120, 0.25, 3)
+    small_oracle = oracle.Graph(small.bits, small.n, small.edges)
+    for algorithm, local_search in (("NeighborhoodCuda", 0), ("NeighborhoodWithManyLocalSearchesCuda", 1)):
+        out = subprocess.run([REF_CLI, "--algorithm", algorithm, "--n", "120", "--density", "0.25", "--graph-seed",
+                              "3", "--k", "2", "--pop", "32"], cwd=tmp_path, capture_output=True, text=True,
+                             timeout=120)
+        assert out.returncode == 0, out.stdout
+        assert objective_line(out.stdout) == orc.neighborhood(small_oracle, local_search=local_search)["objective"]
