@@ -24,20 +24,21 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Memory-fabric bytes of the descent kernel, measured: the mean over the launches of a solve of FETCH_SIZE x 2.0 plus
-# WRITE_SIZE x 1.0 from two rocprofv3 --pmc runs (a 1 GiB device copy reads as FETCH_SIZE 0.5 GiB and WRITE_SIZE
-# 1 GiB on this device, hence the factors). Per config: (iterations of the profiled solve, bytes per launch, bytes per
-# accepted move). A timed solve of the same length has the same trajectory, so its launches move the same bytes;
-# other lengths are scaled by their accepted moves. Commands and counter values: profiles/descent_*.txt.
-MEASURED_TRAFFIC = {
-    "c2": (200, 445444430, 877.9),
-    "c5": (2, 195321610816, 30241.7),
-}
+# Hardware counters of the descent kernels per accepted move, from rocprofv3 --pmc runs of this script (the commands,
+# raw per-dispatch values and their means are in the profiles/ file named beside each entry). Keyed by (config,
+# descent, threads per workgroup, vertices per thread): another plan issues other counts and is reported without
+# them. valu: wave64 VALU instructions (SQ_INSTS_VALU); lds: LDS instructions (SQ_INSTS_LDS); lds_conflict: cycles
+# of LDS bank conflicts (SQ_LDS_BANK_CONFLICT); fabric: bytes moved beyond L2, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (KiB -> bytes; the factors from a 1 GiB device copy, profiles/copy_calibration_1gib.txt).
+MEASURED = {}
 
 METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
-# int32 VALU lanes: 256 CUs x 64 lanes x 2.4 GHz
-VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# MI355X: 256 CUs at 2.4 GHz; a CU issues at most one wave64 VALU instruction per cycle (four SIMDs, four cycles
+# each), i.e. 64 int32 lane operations per CU-cycle.
+CUS = 256
+CLOCK_HZ = 2.4e9
+VALU_PEAK_LANE_OPS = CUS * 64 * CLOCK_HZ
 
 CONFIGS = {
     "c1": dict(workload="ER G(n=1000, p=0.5) graph-seed 1; k=2, pop=128, gww=0.1, seed=42, early stop off",
@@ -97,6 +98,49 @@ def descent_state_bytes(n, k):
     return 2 * (k * n * gbytes + n)
 
 
+def roofline(config, cfg, graph, prof, mean_launch_s, moves_per_launch):
+    """The descent kernel (95 % of the device time) against the roof that binds it. The register and chunk kernels
+    keep an individual's state on chip and issue VALU instructions back to back, so with counters measured for the
+    plan the bound is VALU issue: SQ_INSTS_VALU per move x moves per launch x 64 lanes over the launch time, against
+    256 CUs x 64 lanes x 2.4 GHz; one workgroup holds one individual and takes one CU, so with fewer individuals than
+    CUs only cus_used CUs can issue at all (frac_of_used_cus). Global-memory bytes (algorithmic, and the measured
+    fabric traffic) and the survey's algorithmic lane operations per move, 7 (k - 1) n, are given beside it. Without
+    counters for the plan the line falls back to the algorithmic bytes against HBM."""
+    n, k, pop = graph.n, cfg["k"], cfg["population"]
+    descent = prof["descent_name"]
+    key = (config, descent, prof["threads"], prof["vertices_per_thread"])
+    meas = MEASURED.get(key)
+    bytes_per_launch = (moves_per_launch * descent_bytes_per_move(n, k, descent, 2.0 * graph.edges / n) +
+                        pop * descent_state_bytes(n, k))
+    gbs = bytes_per_launch / mean_launch_s / 1e9 if mean_launch_s > 0 else 0.0
+    hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+    cus_used = min(pop, CUS)
+    out = {"kernel": f"descent ({descent}, {prof['threads']} threads, {prof['vertices_per_thread']} vertices per "
+                     f"thread)",
+           "mean_launch_ms": round(1e3 * mean_launch_s, 4), "moves_per_launch": round(moves_per_launch),
+           "cus_used": cus_used}
+    if meas is None or mean_launch_s <= 0:
+        out.update(bound="hbm", achieved=hbm["achieved"], peak=HBM_PEAK_GBS, unit="GB/s", frac=hbm["frac"],
+                   traffic=None, bytes_per_launch=hbm["algorithmic_bytes_per_launch"],
+                   note="no counters measured for this plan")
+        return out
+    lane_ops = meas["valu"] * moves_per_launch * 64 / mean_launch_s
+    frac = lane_ops / VALU_PEAK_LANE_OPS
+    algo_ops = 7 * (k - 1) * n
+    algo = moves_per_launch * algo_ops / mean_launch_s
+    out.update(bound="valu", achieved=round(lane_ops / 1e12, 3), peak=round(VALU_PEAK_LANE_OPS / 1e12, 2),
+               unit="Tlane-op/s", frac=round(frac, 4), frac_of_used_cus=round(frac * CUS / cus_used, 4),
+               traffic=round(meas["fabric"] * moves_per_launch),
+               valu_instructions_per_move=meas["valu"], lds_instructions_per_move=meas["lds"],
+               lds_bank_conflict_cycles_per_move=meas["lds_conflict"],
+               hbm=dict(hbm, fabric_gbs=round(meas["fabric"] * moves_per_launch / mean_launch_s / 1e9, 1)),
+               algorithmic={"lane_ops_per_move": algo_ops, "achieved": round(algo / 1e12, 3),
+                            "frac": round(algo / VALU_PEAK_LANE_OPS, 4)},
+               counters=meas["profile"])
+    return out
+
+
 def reference_solver():
     import oracle
 
@@ -110,9 +154,34 @@ def same_result(a, b):
             a["iterations_done"] == b["iterations_done"] and bool((a["labels"] == b["labels"]).all()))
 
 
-def cpu_sample(cfg, graph, dg, threads):
-    """The steady-state iteration on the CPU: reference solves of m and m + 1 iterations (m = cpu_iterations),
-    differenced on the solver's own clock; the GPU solves the same two prefixes and must match both bit for bit."""
+def host_cpu():
+    """nproc and the CPU model name of this host (lscpu, else /proc/cpuinfo)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_sample(cfg, graph, dg, threads, repeats=3):
+    """The steady-state iteration on the CPU: after one warm-up solve, `repeats` pairs of reference solves of m and
+    m + 1 iterations (m = cpu_iterations), each pair differenced on the solver's own clock; the median difference is
+    the CPU's time per iteration. The GPU solves the same two prefixes and must match them bit for bit."""
+    import statistics
+
     import oracle
 
     ref = reference_solver()
@@ -121,15 +190,20 @@ def cpu_sample(cfg, graph, dg, threads):
     og = oracle.Graph(graph.bits, graph.n, graph.edges)
     common = dict(k=cfg["k"], population=cfg["population"], gww=cfg["gww"], seed=cfg["seed"])
     m = cfg["cpu_iterations"]
-    out = {"iterations": m, "bit_exact": True}
-    for it in (m, m + 1):
-        cpu = ref.solve_pbils(og, iterations=it, early_stop=it, threads=threads, **common)
-        gpu = dg.solve_pbils(iterations=it, early_stop=it, **common)
-        out[f"cpu_{it}"] = cpu["seconds"]
-        out[f"gpu_{it}"] = gpu["seconds"]
-        out["bit_exact"] &= same_result(cpu, gpu)
-        note(f"cpu sample: {it} iterations, reference {cpu['seconds']:.2f} s")
-    out["cpu_iteration_s"] = out[f"cpu_{m + 1}"] - out[f"cpu_{m}"]
+    ref.solve_pbils(og, iterations=m, early_stop=m, threads=threads, **common)  # warm-up: pages, caches, threads
+    out = {"iterations": m, "bit_exact": True, "pairs": []}
+    for rep in range(repeats):
+        secs = {}
+        for it in (m, m + 1):
+            cpu = ref.solve_pbils(og, iterations=it, early_stop=it, threads=threads, **common)
+            secs[it] = cpu["seconds"]
+            if rep == 0:
+                gpu = dg.solve_pbils(iterations=it, early_stop=it, **common)
+                out[f"gpu_{it}"] = gpu["seconds"]
+                out["bit_exact"] &= same_result(cpu, gpu)
+        out["pairs"].append([round(secs[m], 4), round(secs[m + 1], 4)])
+        note(f"cpu sample {rep + 1}/{repeats}: {m} and {m + 1} iterations, {secs[m]:.2f} s and {secs[m + 1]:.2f} s")
+    out["cpu_iteration_s"] = statistics.median(b - a for a, b in out["pairs"])
     out["gpu_iteration_s"] = out[f"gpu_{m + 1}"] - out[f"gpu_{m}"]
     return out
 
@@ -477,26 +551,25 @@ def main():
             note(f"autotune: {len(cands)} plans, chose {cands[chosen]['descent']} {cands[chosen]['threads']} threads")
         if args.warmup > 0:
             dg.solve_pbils(iterations=args.warmup, early_stop=args.warmup, **common)
+        # The timed solve runs as a user's would: iterations replayed from the captured HIP graph, no event timing.
         t0 = time.perf_counter()
-        res = dg.solve_pbils(iterations=steps, early_stop=steps, profile=True, **common)
+        res = dg.solve_pbils(iterations=steps, early_stop=steps, **common)
         wall = time.perf_counter() - t0
         note(f"{args.config}: {res['iterations_done']} iterations in {wall:.3f} s")
+        # The kernel times come from a second solve of the same trajectory with HIP events around every launch
+        # (launched one by one on the library's stream); it must reproduce the timed solve bit for bit.
+        prof = dg.solve_pbils(iterations=steps, early_stop=steps, profile=True, **common)
+        if not same_result(res, prof):
+            raise SystemExit("the profiled solve did not reproduce the timed one")
         sample = None if args.no_cpu else cpu_sample(cfg, graph, dg, args.cpu_threads)
 
     n, k = graph.n, cfg["k"]
     iters = res["iterations_done"]
     value = iters / wall
-    launches = max(1, res["search_launches"])
-    mean_launch_s = res["search_ms"] / 1e3 / launches
-    moves_per_launch = res["accepted_moves"] / launches
-    descent = res["descent_name"]
-    bytes_per_launch = (moves_per_launch * descent_bytes_per_move(n, k, descent, 2.0 * graph.edges / n) +
-                        cfg["population"] * descent_state_bytes(n, k))
-    traffic = None
-    if args.config in MEASURED_TRAFFIC:
-        t_steps, t_launch, t_move = MEASURED_TRAFFIC[args.config]
-        traffic = t_launch if t_steps == steps else round(t_move * moves_per_launch)
-    achieved = bytes_per_launch / mean_launch_s / 1e9 if mean_launch_s > 0 else 0.0
+    launches = max(1, prof["search_launches"])
+    mean_launch_s = prof["search_ms"] / 1e3 / launches
+    moves_per_launch = prof["accepted_moves"] / launches
+    descent = prof["descent_name"]
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -511,46 +584,30 @@ def main():
         "dtype": "int16" if n <= 32767 else "int32",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "n": n, "edges": graph.edges, "k": k,
-                   "population": cfg["population"], "iterations": steps, "descent": res["descent_name"],
-                   "descent_threads": res["threads"]},
+                   "population": cfg["population"], "iterations": steps, "descent": descent,
+                   "descent_threads": prof["threads"], "vertices_per_thread": prof["vertices_per_thread"]},
         "moves_per_s": round(res["accepted_moves"] / wall, 1),
         "accepted_moves": res["accepted_moves"],
         "objective": res["objective"],
-        "device_ms": {"descent": round(res["search_ms"], 3), "rebuild": round(res["rebuild_ms"], 3),
-                      "iterations": round(res["iteration_ms"], 3)},
-        "roofline": {
-            "kernel": "descent (" + res["descent_name"] + ")",
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": round(bytes_per_launch),
-            "mean_launch_ms": round(1e3 * mean_launch_s, 4),
-        },
+        "device_ms": {"descent": round(prof["search_ms"], 3), "rebuild": round(prof["rebuild_ms"], 3),
+                      "iterations": round(prof["iteration_ms"], 3), "descent_launches": launches},
+        "roofline": roofline(args.config, cfg, graph, prof, mean_launch_s, moves_per_launch),
     }
-    if descent in REGISTER_DESCENTS and mean_launch_s > 0:
-        # The state never leaves the CU, so HBM is far from binding; the roof is VALU issue. Algorithmic lane
-        # operations per move: 7 (k - 1) n (scan every (vertex, target) pair and update the neighbours).
-        ops = 7 * (k - 1) * n
-        valu = moves_per_launch * ops / mean_launch_s
-        out["roofline"]["valu"] = {
-            "achieved": round(valu / 1e12, 2), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2), "unit": "Tlane-op/s",
-            "frac": round(valu / VALU_PEAK_LANE_OPS, 4), "ops_per_move": ops,
-            "cus_used": min(cfg["population"], 256)}
     if sample is not None:
         m = sample["iterations"]
         cpu_rate = 1.0 / sample["cpu_iteration_s"]
+        host = host_cpu()
         out["cpu_baseline"] = {
             "value": round(cpu_rate, 5),
             "unit": "iter/s",
             "cores": args.cpu_threads,
             "kind": "reference",
-            "sample": f"iteration {m + 1} of the same workload (tournament, 128 descents from perturbed optima, "
-                      f"record, GWW, perturb, rebuild): reference SolveCpu with {args.cpu_threads} threads, "
-                      f"solves of {m + 1} and {m} iterations differenced on its own clock",
-            "cpu_seconds": {str(m): round(sample[f"cpu_{m}"], 3), str(m + 1): round(sample[f"cpu_{m + 1}"], 3)},
+            "sample": f"iteration {m + 1} of the same workload (tournament, {cfg['population']} descents from "
+                      f"perturbed optima, record, GWW, perturb, rebuild): reference SolveCpu with "
+                      f"{args.cpu_threads} threads; after a warm-up solve, the median over {len(sample['pairs'])} "
+                      f"pairs of solves of {m} and {m + 1} iterations differenced on its own clock",
+            "pairs_s": sample["pairs"],
+            "host": host,
             "gpu_same_iteration_s": round(sample["gpu_iteration_s"], 5),
             "gpu_over_cpu": round(value / cpu_rate, 1),
             "bit_exact": sample["bit_exact"],

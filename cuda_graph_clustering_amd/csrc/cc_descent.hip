@@ -384,9 +384,10 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
   }
 }
 
-// The register descent's move term of vertex slot J, m = 4096 - 8192 A[v][slot] = (bit J of `bits`, sign-extended,
-// & -8192) | 4096, as v_bfe_i32 and one v_and_or_b32. gfx950's three-operand encoding takes no literal, so the mask
-// comes in a VGPR and 4096 in an SGPR (left to itself the compiler spends four instructions).
+// The register descent's move term of vertex slot J, m = 2^Q - 2^(Q+1) A[v][slot] = (bit J of `bits`, sign-extended,
+// & mask) | bias with mask = -2^(Q+1) and bias = 2^Q, as v_bfe_i32 and one v_and_or_b32. gfx950's three-operand
+// encoding takes no literal, so the mask comes in a VGPR and the bias in an SGPR (left to itself the compiler spends
+// four instructions).
 template <int J>
 __device__ __forceinline__ int MoveTerm(uint32_t bits, int mask, int bias) {
   int r;
@@ -400,23 +401,29 @@ constexpr int RegsLabelBits(int k) { return k <= 4 ? 2 : k <= 16 ? 4 : 5; }
 // extra clusters stay empty and are never offered as targets.
 constexpr int RegsClusters(int k) { return k <= 16 ? k : (k + 3) & ~3; }
 
+// Bit position of q in the register descent's keys: 12, or 13 when the slot index needs a fifth bit above the two
+// label fields (the 512-thread instances with up to 32 slots per thread). With |q| <= 2n < 2^16 a key stays within
+// 2^29 either way.
+constexpr int RegsQShift(int k, int vpt) { return (vpt << (2 * RegsLabelBits(k))) <= 4096 ? 12 : 13; }
+
 // k in [3, 32], narrow path, per-vertex state in registers. Thread t owns the kVpt consecutive vertices
 // v = t kVpt + j, so the row bits of a move that it needs are one 64-bit window of the row (two dword loads and one
 // v_alignbit) and a wave's loads share one or two cache lines. With b the label field width (RegsLabelBits: 2 bits
-// up to k = 4, 4 up to 16, 5 above) the slot index j sits at bit 2b and stays below bit 12.
+// up to k = 4, 4 up to 16, 5 above) the slot index j sits at bit 2b and stays below bit Q = RegsQShift (12, or 13
+// for the widest slot counts).
 //
 // With q_c(v) = s_c - 2 G[v][c], the best move of v goes to the cluster of smallest q_c (ties to the smaller c) and
 // has delta = q_c - q_a + 1 (a = label of v). Per (j, c) one register holds y = -key with
-//   key(j, c) = (q_c << 12) | (j << 2b) | (a << b) | c,
-// and per j the own term qa = q_a << 12. Then z_j = max_c y[j][c] + qa = -x_j with
-//   x_j = ((q_c - q_a) << 12) | (j << 2b) | (a << b) | c,
+//   key(j, c) = (q_c << Q) | (j << 2b) | (a << b) | c,
+// and per j the own term qa = q_a << Q. Then z_j = max_c y[j][c] + qa = -x_j with
+//   x_j = ((q_c - q_a) << Q) | (j << 2b) | (a << b) | c,
 // whose signed order is (delta, j, c): the largest z over the thread's vertices is its smallest-delta move, ties to
 // its smaller vertex, and the target comes with it. The scan is thus ceil((K - 1) / 2) v_max3 and one add per
 // vertex. Including c = a in the maximum is harmless: it gives delta 1, never an improving move. The workgroup
 // minimum of ((delta - 1 + 65535) << 15) | v (32 bits, n <= 32767) is the reference's move order (delta, v); the
 // winner's label and target are read from the owner's entry of an LDS table (cand[]).
 //
-// A move v: from -> to changes, with bit = [u is a neighbour of v] and m = 4096 - 8192 bit,
+// A move v: from -> to changes, with bit = [u is a neighbour of v] and m = 2^Q - 2^(Q+1) bit,
 //   y[from] += m,  y[to] -= m        (s_from - 1, s_to + 1, G[u][from] - bit, G[u][to] + bit)
 //   qa += t_a m,   t_a = [a == to] - [a == from]   (fetched per lane with ds_bpermute from a lane table)
 // for every vertex u; the mover then rewrites its own label bits and qa. Lanes beyond n hold y = -2^30, qa = 0 and
@@ -426,7 +433,6 @@ constexpr int RegsClusters(int k) { return k <= 16 ? k : (k + 3) & ~3; }
 template <int T, int K, int kVpt>
 __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  constexpr int kWaves = T / kWave;
   constexpr int kPadY = -(1 << 30);
   constexpr int kPadLabel = 63;
   // Cache warming of the candidates' rows (below): before the barrier for k in [5, 11], after the winner's row loads
@@ -438,15 +444,17 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   constexpr int kLb = RegsLabelBits(K);
   constexpr int kLabelMask = (1 << kLb) - 1;
   constexpr int kSlotShift = 2 * kLb;
+  constexpr int kQ = RegsQShift(K, kVpt);
+  constexpr int kUnit = 1 << kQ;  // one unit of q in a key
   using Cand = std::conditional_t<(kSlotShift <= 8), uint8_t, uint16_t>;
   static_assert(K >= 3 && K <= 32 && RegsClusters(K) == K, "register descent holds 3 to 16 clusters, or 20 to 32 by 4");
-  static_assert((kVpt << kSlotShift) <= 4096 && kVpt * T <= 32768, "slot index below bit 12, vertex in 15 bits");
+  static_assert((kVpt << kSlotShift) <= kUnit && kVpt * T <= 32768, "slot index below bit Q, vertex in 15 bits");
   static_assert(kVpt <= 32, "a thread's row bits lie in one 64-bit window");
   const int n = r.n;
   const int k = r.k;  // == K up to 16; above, K - k empty clusters pad the registers
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  __shared__ uint32_t partial[2][kWaves];
+  __shared__ uint32_t partial[2][T / kWave];
   __shared__ Cand cand[2][T];
   __shared__ int32_t src_sh;
   if (tid == 0) {
@@ -480,9 +488,9 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       if ((c >> 2) == i) szp[i] += inc;  // uniform
     }
   };
-  int kc[K];  // (s_c << 12) + c at the start
+  int kc[K];  // (s_c << Q) + c at the start
 #pragma unroll
-  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
+  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << kQ) + c;
 
   int y[kVpt][K];
   int qa[kVpt];
@@ -500,11 +508,11 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
 #pragma unroll
     for (int c = 0; c < K; ++c) {
       const int g = live(c) ? g_in[static_cast<size_t>(c) * n + vc] : 0;
-      y[j][c] = real && live(c) ? (g << 13) - (j << kSlotShift) - (a << kLb) - kc[c] : kPadY;
+      y[j][c] = real && live(c) ? (g << (kQ + 1)) - (j << kSlotShift) - (a << kLb) - kc[c] : kPadY;
       ga = c == a ? g : ga;
       sa = c == a ? size_of(c) : sa;
     }
-    qa[j] = real ? (sa << 12) - (ga << 13) : 0;
+    qa[j] = real ? (sa << kQ) - (ga << (kQ + 1)) : 0;
     lab4[j] = (real ? a : kPadLabel) * 4;
   }
   const int64_t f_start = Uniform(r.pop.f[src]);
@@ -522,11 +530,11 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       for (int c = 1; c < K; ++c) ymax = max(ymax, y[j][c]);
       zmax = max(zmax, ymax + qa[j]);
     }
-    const int x = -zmax;  // ((delta - 1) << 12) | (j << 2b) | (a << b) | c
+    const int x = -zmax;  // ((delta - 1) << Q) | (j << 2b) | (a << b) | c
     cand[parity][tid] = static_cast<Cand>(x & ((1 << kSlotShift) - 1));
-    if (x >= -4096) return ~0u;  // delta >= 0: no improving move
-    const int j = (x >> kSlotShift) & ((4096 >> kSlotShift) - 1);
-    return (static_cast<uint32_t>((x >> 12) + 65535) << 15) | static_cast<uint32_t>(first + j);
+    if (x >= -kUnit) return ~0u;  // delta >= 0: no improving move
+    const int j = (x >> kSlotShift) & ((kUnit >> kSlotShift) - 1);
+    return (static_cast<uint32_t>((x >> kQ) + 65535) << 15) | static_cast<uint32_t>(first + j);
   };
 
   int parity = 0;
@@ -537,8 +545,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   const int window = first >> 5;  // the dword holding bit `first` of a row; the window's second dword follows
   const uint32_t shift = static_cast<uint32_t>(first & 31);
   uint32_t warm = 0;  // the previous move's cache-warming load, consumed one move later
+  // The wave minimum through lane swaps measured 1.0 % faster per launch than through four v_readlane (k = 5,
+  // n = 10000, 20 iterations: 5.89 -> 5.83 ms). A workgroup minimum through an LDS atomic (ds_min_u32 into one of
+  // three rotating words, no reduction after the barrier) gained 0.8 % alone but lost 3 % together with the swaps,
+  // so the per-wave partials stay.
   while (true) {
-    const uint32_t wave_best = WaveMin(key);
+    const uint32_t wave_best = WaveMinSwap(key);
     // Each wave touches the cache lines of its own best candidate's row (one dword per 128-byte line) before the
     // barrier: the winner is one of these candidates, so its row is on its way while the workgroup minimum is
     // formed, and the runners-up are often the next moves' winners. The value is consumed, without effect, one move
@@ -551,7 +563,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
       warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     }
-    const uint32_t best = BlockMinOfWaves<kWaves>(wave_best, partial[parity]);
+    const uint32_t best = BlockMinOfWaves<T / kWave>(wave_best, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
@@ -573,7 +585,9 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     parity ^= 1;
     size_add(to, 1);
     size_add(from, -1);
-    // lane c of the table holds t_c = [c == to] - [c == from]
+    // lane c of the table holds t_c = [c == to] - [c == from]. (Issuing all the lookups before the first use of
+    // the row, held in place by a scheduling barrier, measured 1 % slower at k = 5, n = 10000: the row has usually
+    // arrived by then, and the compiler's own interleaving of lookups and neighbour terms is the better one.)
     const int table = (lane == to) - (lane == from);
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     if constexpr (kWarmEarly || kWarmLate) {
@@ -587,12 +601,12 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       // (C2, ten slots: 1.6 %; sixteen: 12 %, with less register pressure), 2-5 % slower with up to four, where
       // the compiler's own form schedules better around the short move.
       if constexpr (kVpt >= 6) {
-        m[j] = MoveTerm<j>(bits, ~8191, 4096);
+        m[j] = MoveTerm<j>(bits, ~(2 * kUnit - 1), kUnit);
       } else {
-        m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1) << 13) + 4096;
+        m[j] = (__builtin_amdgcn_sbfe(static_cast<int>(bits), j, 1) << (kQ + 1)) + kUnit;
       }
       const int ta = __builtin_amdgcn_ds_bpermute(lab4[j], table);
-      qa[j] += __mul24(ta, m[j]);  // |ta| <= 1, |m| = 4096: a full-rate 24-bit multiply
+      qa[j] += __mul24(ta, m[j]);  // |ta| <= 1, |m| = 2^Q: a full-rate 24-bit multiply
     });
 #pragma unroll
     for (int c = 0; c < K; ++c) {
@@ -605,7 +619,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
         for (int j = 0; j < kVpt; ++j) y[j][c] -= m[j];
       }
     }
-    // The moved vertex: new label bits in its y, and its own term qa = (s_to << 12) - (G_to << 13), which is
+    // The moved vertex: new label bits in its y, and its own term qa = (s_to << Q) - (G_to << (Q + 1)), which is
     // -(y[to] + (j << 2b) + (to << b) + to) once y carries the new sizes and label.
     const int mv_j = v - owner * kVpt;
     if (tid == owner) {
@@ -623,8 +637,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
         }
       }
     }
-    // Row words of slots beyond n are not masked: a padding lane's y (kPadY) drifts by at most 4096 per move, so
-    // it is reset every 4096 moves, long before it could come near a real entry (|y| < 2^29).
+    // Row words of slots beyond n are not masked: a padding lane's y (kPadY) drifts by at most 2^Q per move, so
+    // it is reset every 4096 moves (at most 2^25 of drift), long before it could come near a real entry (|y| < 2^29).
     if ((applied & 4095) == 4095) {
 #pragma unroll
       for (int j = 0; j < kVpt; ++j) {
@@ -648,7 +662,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int16_t* g_out = static_cast<int16_t*>(r.next.g) + static_cast<size_t>(s) * k * n;
 #pragma unroll
-  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << 12) + c;
+  for (int c = 0; c < K; ++c) kc[c] = (size_of(c) << kQ) + c;
 #pragma unroll
   for (int j = 0; j < kVpt; ++j) {
     const int v = first + j;
@@ -659,7 +673,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       for (int c = 0; c < K; ++c) {
         if (live(c)) {
           g_out[static_cast<size_t>(c) * n + v] =
-              static_cast<int16_t>((y[j][c] + kc[c] + (j << kSlotShift) + (a << kLb)) >> 13);
+              static_cast<int16_t>((y[j][c] + kc[c] + (j << kSlotShift) + (a << kLb)) >> (kQ + 1));
         }
       }
     }
@@ -1543,17 +1557,21 @@ bool GoK2Any(const Go& go) {
   }
 }
 
-// Register descent instances: T = 256 with kVpt in {1, 2, 4}; T = 1024 with kVpt in {2, 4, 6, 8, 10, 12},
-// bounded by kVpt (K + 5) <= 112 so that the state fits the 128 VGPRs of a 1024-thread workgroup.
-// Registers of a register-descent thread: vpt (K + 3) for the slot state, within the 128 VGPRs of 16 waves per CU;
-// the slot index stays below bit 12 of the key.
-constexpr bool RegsFit(int clusters, int vpt) {
+// Register descent instances: T = 256 with kVpt in {1, 2, 4}; T = 1024 with kVpt in {2, 4, ..., 16}; T = 512 with
+// kVpt in {8, 12, ..., 32} for k up to 16. A thread keeps vpt (K + 3) registers of slot state: at most 104 of the 128
+// VGPRs a wave has with 16 waves per CU (256 and 1024 threads), at most 200 of the 256 with 8 waves (512 threads,
+// which trade half the waves and their per-move reductions for twice the slots per thread). The slot index stays
+// below bit 12 of the key, or bit 13 in the 512-thread instances (RegsQShift).
+constexpr bool RegsFit(int clusters, int vpt, int threads = 1024) {
+  if (threads == 512) {
+    return clusters <= 16 && vpt * (clusters + 3) <= 200 && (vpt << (2 * RegsLabelBits(clusters))) <= 8192;
+  }
   return vpt * (clusters + 3) <= 104 && (vpt << (2 * RegsLabelBits(clusters))) <= 4096;
 }
 
 template <int T, int K, int kVpt>
 bool GoRegsOne(const Go& go) {
-  if constexpr (RegsFit(K, kVpt)) {
+  if constexpr (RegsFit(K, kVpt, T)) {
     return go.Run(KernelDescentRegs<T, K, kVpt>, T, 0);
   } else {
     return false;
@@ -1567,6 +1585,18 @@ bool GoRegsK(const Go& go) {
       case 1: return GoRegsOne<256, K, 1>(go);
       case 2: return GoRegsOne<256, K, 2>(go);
       case 4: return GoRegsOne<256, K, 4>(go);
+      default: return false;
+    }
+  }
+  if (go.plan->threads == 512) {
+    switch (go.plan->vpt) {
+      case 8: return GoRegsOne<512, K, 8>(go);
+      case 12: return GoRegsOne<512, K, 12>(go);
+      case 16: return GoRegsOne<512, K, 16>(go);
+      case 20: return GoRegsOne<512, K, 20>(go);
+      case 24: return GoRegsOne<512, K, 24>(go);
+      case 28: return GoRegsOne<512, K, 28>(go);
+      case 32: return GoRegsOne<512, K, 32>(go);
       default: return false;
     }
   }

@@ -70,8 +70,8 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
     }
   }
   if (!wide && k >= 3 && k <= 32) {
-    for (int t : {256, 1024}) {
-      for (int v : {1, 2, 4, 6, 8, 10, 12, 14, 16}) {
+    for (int t : {256, 512, 1024}) {
+      for (int v : {1, 2, 4, 6, 8, 10, 12, 14, 16, 20, 24, 28, 32}) {
         if (v * t < n) continue;
         DescentPlan p = base;
         p.descent = Descent::kRegisters;

@@ -345,3 +345,22 @@ def test_packed_k3_kernel(cga, orc, cpu_threads, n, vpt):
     got, want = run_both(cga, orc, g, cpu_threads, **kw)
     check_same(got, want, f"packed k = 3, n={n}")
     assert (got["descent_name"], got["threads"], got["vertices_per_thread"]) == ("k3-packed", 1024, vpt)
+
+
+@pytest.mark.parametrize("n,k,vpt", [(10000, 5, 20), (12000, 5, 24), (9000, 7, 20), (6000, 8, 12), (16000, 3, 32),
+                                     (14000, 4, 28), (4000, 16, 8)])
+def test_register_kernel_with_512_threads(cga, orc, cpu_threads, tmp_path, n, k, vpt):
+    """The 512-thread register descent (eight waves, up to 32 vertex slots per thread; from five slot bits on, the
+    keys carry q from bit 13 instead of 12) forced through a tuned-plan entry: it must follow the oracle exactly."""
+    g = cga.sbm(n, 4, 0.3, 0.05, 31 + k)
+    kw = dict(k=k, population=4, iterations=2, early_stop=2, seed=k, gww=0.1)
+    table = tmp_path / "plan.txt"
+    table.write_text(f"{n} {k} 4 0 5 512 {vpt} 0 0 0\n")
+    cga.tune_clear()
+    try:
+        assert cga.tune_load(table) == 1
+        got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    finally:
+        cga.tune_clear()
+    check_same(got, want, f"512-thread register kernel n={n} k={k} vpt={vpt}")
+    assert (got["descent_name"], got["threads"], got["vertices_per_thread"]) == ("registers", 512, vpt)
