@@ -125,6 +125,7 @@ ABI_SYMBOLS = [
     "cc_hip_graph_upload",
     "cc_hip_graph_free",
     "cc_hip_graph_solve_pbils",
+    "cc_hip_graph_solve_pbils_static",
     "cc_hip_graph_solve_neighborhood",
     "cc_hip_solve_batch",
     "cc_serve_default_options",
@@ -168,6 +169,9 @@ def _load():
     lib.cc_hip_graph_free.argtypes = [ctypes.c_void_p]
     lib.cc_hip_graph_solve_pbils.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), _i32p, ctypes.POINTER(Stats),
                                              _i64p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_graph_solve_pbils_static.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_void_p, _i32p,
+                                                    ctypes.POINTER(Stats), _i64p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.cc_hip_graph_solve_pbils_static.restype = ctypes.c_int
     lib.cc_hip_graph_solve_neighborhood.argtypes = [ctypes.c_void_p, ctypes.POINTER(Params), ctypes.c_int32, _i32p,
                                                     ctypes.POINTER(Stats), ctypes.c_char_p, ctypes.c_size_t]
     lib.cc_hip_solve_batch.argtypes = [ctypes.POINTER(BatchRun), ctypes.c_int32, ctypes.c_int32,
@@ -416,6 +420,20 @@ class DeviceGraph:
         err = ctypes.create_string_buffer(512)
         rc = lib.cc_hip_graph_solve_pbils(self.handle, ctypes.byref(p), labels.ctypes.data_as(_i32p),
                                           ctypes.byref(stats), trace.ctypes.data_as(_i64p), err, 512)
+        _check(rc, err)
+        return _result(stats, labels, self.n, trace)
+
+    def solve_pbils_static(self, stream=None, **kw):
+        """solve_pbils without any device allocation: the run's state lives in the library's static scratch region
+        (cc_hip_graph_solve_pbils_static); `stream` is a hipStream_t handle as an int, or None for the default
+        stream. Calls that overlap on one GPU race."""
+        p = make_params(**kw)
+        labels = np.zeros(max(self.n, 1), dtype=np.int32)
+        trace = np.zeros(max(p.iterations, 1), dtype=np.int64)
+        stats = Stats()
+        err = ctypes.create_string_buffer(512)
+        rc = lib.cc_hip_graph_solve_pbils_static(self.handle, ctypes.byref(p), stream, labels.ctypes.data_as(_i32p),
+                                                 ctypes.byref(stats), trace.ctypes.data_as(_i64p), err, 512)
         _check(rc, err)
         return _result(stats, labels, self.n, trace)
 

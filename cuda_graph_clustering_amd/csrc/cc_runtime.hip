@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -111,6 +112,17 @@ size_t DevicePool::in_use() const {
   return in_use_;
 }
 
+void* ScratchArena::Take(size_t bytes) {
+  const size_t need = (bytes + 255) & ~static_cast<size_t>(255);
+  if (need > bytes_ - used_) {
+    Fail(CC_HIP_ENOMEM, "the static scratch region of %s is exhausted (%s taken, %s more needed)",
+         Mebibytes(bytes_).c_str(), Mebibytes(used_).c_str(), Mebibytes(need).c_str());
+  }
+  void* p = base_ + used_;
+  used_ += need;
+  return p;
+}
+
 DeviceMemory::~DeviceMemory() {
   for (void* p : blocks_) {
     if (pool_) {
@@ -123,6 +135,7 @@ DeviceMemory::~DeviceMemory() {
 
 void* DeviceMemory::AllocateBytes(size_t bytes) {
   if (bytes == 0) return nullptr;
+  if (arena_) return arena_->Take(bytes);  // not recorded: an arena's blocks are never released
   void* p = nullptr;
   if (pool_) {
     p = pool_->Take(bytes);
@@ -404,9 +417,11 @@ size_t RunState::Footprint(const RunSpec& spec) {
   return a.used();
 }
 
-RunState::RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool) : spec_(spec) {
+RunState::RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool, ScratchArena* arena) : spec_(spec) {
   bytes_ = Footprint(spec_);
-  if (pool) {
+  if (arena) {
+    memory_.UseArena(arena);
+  } else if (pool) {
     memory_.UsePool(pool);
   } else {
     const size_t free_bytes = FreeDeviceBytes();
@@ -418,10 +433,10 @@ RunState::RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool) : 
   uint8_t* base = static_cast<uint8_t*>(memory_.AllocateBytes(bytes_));
   // Start from zeros: e.g. a Neighborhood batch without local search sums a moves[] that no descent writes.
   CC_HIP_CHECK(hipMemsetAsync(base, 0, bytes_, stream));
-  Arena arena;
-  arena.Start(base);
+  Arena layout;
+  layout.Start(base);
   RunDev& r = dev_;
-  LayoutRun(spec_, r, arena);
+  LayoutRun(spec_, r, layout);
   const cc_pbils_params& p = spec_.params;
   r.bits = spec_.graph->bits;
   r.deg = spec_.graph->deg;
@@ -444,11 +459,16 @@ RunState::RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool) : 
   r.variant = spec_.variant;
 }
 
-LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, DevicePool* pool, PinnedBuffer* pinned)
+LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, DevicePool* pool, PinnedBuffer* pinned,
+                     ScratchArena* arena)
     : states_(states), stream_(stream) {
   const int nruns = static_cast<int>(states_.size());
   if (nruns == 0) Fail(CC_HIP_EINVAL, "empty batch");
-  if (pool) memory_.UsePool(pool);
+  if (arena) {
+    memory_.UseArena(arena);
+  } else if (pool) {
+    memory_.UsePool(pool);
+  }
   host_runs_.resize(nruns);
   for (int i = 0; i < nruns; ++i) {
     host_runs_[i] = states_[i]->dev();
@@ -490,9 +510,9 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
   };
   Arena sizing;
   layout(sizing);
-  Arena arena;
-  arena.Start(static_cast<uint8_t*>(memory_.AllocateBytes(sizing.used())));
-  layout(arena);
+  Arena tables;
+  tables.Start(static_cast<uint8_t*>(memory_.AllocateBytes(sizing.used())));
+  layout(tables);
 
   CC_HIP_CHECK(hipMemcpyAsync(runs_, host_runs_.data(), sizeof(RunDev) * nruns, hipMemcpyHostToDevice, stream_));
   all_ = ItemList{all_table, static_cast<int32_t>(all.size()), nullptr};
@@ -520,7 +540,10 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
                                 hipMemcpyHostToDevice, stream_));
     descent_.push_back(d);
   }
-  if (pinned) {
+  if (arena) {
+    pageable_ctl_.resize(nruns);
+    host_ctl_ = pageable_ctl_.data();
+  } else if (pinned) {
     host_ctl_ = static_cast<Ctl*>(pinned->Reserve(sizeof(Ctl) * nruns));
   } else {
     CC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), sizeof(Ctl) * nruns, hipHostMallocDefault));
@@ -531,6 +554,34 @@ LaunchSet::LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, D
 
 LaunchSet::~LaunchSet() {
   if (host_ctl_ && own_host_ctl_) (void)hipHostFree(host_ctl_);
+}
+
+// The table layout of the constructor, counted from the specs alone: the run table, all items, the rebuild tiles of
+// each G width, the descent items of each kernel instance, the queue counters and the control-block table.
+size_t LaunchSet::TableBytes(const std::vector<RunSpec>& specs) {
+  size_t all = 0;
+  std::map<int, size_t> tiles;
+  std::vector<std::pair<DescentPlan, size_t>> groups;
+  for (const RunSpec& s : specs) {
+    all += static_cast<size_t>(s.population);
+    tiles[s.plan.wide] += (static_cast<size_t>(s.population) * s.params.k + kRebuildCols - 1) / kRebuildCols;
+    size_t g = 0;
+    while (g < groups.size() && !SameLaunch(groups[g].first, s.plan)) ++g;
+    if (g == groups.size()) groups.push_back({s.plan, 0});
+    groups[g].second += static_cast<size_t>(s.population);
+  }
+  Arena a;
+  RunDev* runs = nullptr;
+  Item* items = nullptr;
+  int32_t* queues = nullptr;
+  Ctl* ctl = nullptr;
+  a.Take(runs, specs.size());
+  a.Take(items, all);
+  for (const auto& kv : tiles) a.Take(items, kv.second);
+  for (const auto& g : groups) a.Take(items, g.second);
+  a.Take(queues, std::max<size_t>(groups.size(), 1));
+  a.Take(ctl, specs.size());
+  return a.used();
 }
 
 void LaunchSet::Rebuild(Timer* timer, cc_pbils_stats* stats) {
@@ -613,14 +664,14 @@ DescentPlan LaunchSet::Plan(int run) const {
   return states_[run]->spec().plan;
 }
 
-Batch::Batch(const std::vector<RunSpec>& specs, hipStream_t stream) : stream_(stream) {
+Batch::Batch(const std::vector<RunSpec>& specs, hipStream_t stream, ScratchArena* arena) : stream_(stream) {
   std::vector<RunState*> raw;
   for (const RunSpec& s : specs) {
-    states_.push_back(std::make_unique<RunState>(s, stream));
+    states_.push_back(std::make_unique<RunState>(s, stream, nullptr, arena));
     raw.push_back(states_.back().get());
     bytes_ += states_.back()->bytes();
   }
-  set_ = std::make_unique<LaunchSet>(raw, stream);
+  set_ = std::make_unique<LaunchSet>(raw, stream, nullptr, nullptr, arena);
 }
 
 Batch::~Batch() {
@@ -757,6 +808,60 @@ void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_
   span.End(stream.get(), &stats->iteration_ms, &spans);
   const Ctl c = batch.PullCtl()[0];
   span.Collect();
+  if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
+  batch.PullLabels(0, labels_out);
+  batch.PullTrace(0, c.iterations_done, trace_out);
+  FillStats(c, spec, labels_out, stats);
+  FillPlan(batch.Plan(0), stats);
+}
+
+// ---- the solve that allocates nothing ------------------------------------------------------------------------
+
+namespace {
+
+// The static scratch region: every piece of device state of SolvePbilsStatic (populations, G, record, trace, GWW and
+// rebuild scratch, launch tables, queue counters). Each GPU holds its own copy at its own address, from the load of
+// this library's code object until the process ends. In host code `scratch` names a host-side stand-in, never the
+// device memory: the device address of the current GPU's copy comes from hipGetSymbolAddress, once per device.
+constexpr size_t kStaticScratchBytes = CC_HIP_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+std::mutex scratch_mutex;
+std::map<int, void*> scratch_address;  // device -> address of its copy of `scratch`
+
+// The current device's copy of the region (the caller has made `device` current).
+void* ScratchAddress(int device) {
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  const auto found = scratch_address.find(device);
+  if (found != scratch_address.end()) return found->second;
+  void* p = nullptr;
+  CC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) Fail(CC_HIP_EDEVICE, "no device address for the static scratch region on device %d", device);
+  scratch_address[device] = p;
+  return p;
+}
+
+}  // namespace
+
+// SolvePbils without an allocation: the run's state and launch tables are laid out in the static scratch region
+// (zeroed by the run state's own hipMemsetAsync), iterations are launched one by one on `stream` (instantiating a HIP
+// graph would allocate), the control blocks come back into pageable host memory, and no events are created.
+void SolvePbilsStatic(const DeviceGraph& g, const cc_pbils_params& p, hipStream_t stream, int32_t* labels_out,
+                      cc_pbils_stats* stats, int64_t* trace_out, Clock::time_point started) {
+  CheckParams(p);
+  if (p.profile) Fail(CC_HIP_EINVAL, "the solve without allocation takes no event timings (params.profile)");
+  const RunSpec spec{&g, p, PlanDescent(g.n, p.k, p), p.population, true};
+  const size_t need = RunState::Footprint(spec) + LaunchSet::TableBytes({spec});
+  if (need > kStaticScratchBytes) {
+    Fail(CC_HIP_ENOMEM, "the run needs %s of device state, more than the %s static scratch region", Mebibytes(need).c_str(),
+         Mebibytes(kStaticScratchBytes).c_str());
+  }
+  ScratchArena arena(ScratchAddress(g.device), kStaticScratchBytes);
+  Batch batch({spec}, stream, &arena);
+  if (p.verbose) PrintPlan(batch.Plan(0), batch.Bytes());
+  batch.Start();
+  RunToCompletion(batch, {spec}, p.verbose != 0, false, p.time_limit_sec, nullptr, stats, started);
+  const Ctl c = batch.PullCtl()[0];
   if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
   batch.PullLabels(0, labels_out);
   batch.PullTrace(0, c.iterations_done, trace_out);
@@ -938,6 +1043,19 @@ int cc_hip_graph_solve_pbils(cc_hip_graph* graph, const cc_pbils_params* params,
     const auto started = cc::hip::Clock::now();
     CC_HIP_CHECK(hipSetDevice(graph->g.device));
     cc::hip::SolvePbils(graph->g, *params, labels_out, stats, trace_out, started);
+    stats->seconds = cc::hip::Since(started);
+  });
+}
+
+int cc_hip_graph_solve_pbils_static(cc_hip_graph* graph, const cc_pbils_params* params, void* stream,
+                                    int32_t* labels_out, cc_pbils_stats* stats, int64_t* trace_out, char* err,
+                                    size_t err_len) {
+  std::memset(stats, 0, sizeof(*stats));
+  return cc::hip::Guard(err, err_len, [&] {
+    const auto started = cc::hip::Clock::now();
+    CC_HIP_CHECK(hipSetDevice(graph->g.device));
+    cc::hip::SolvePbilsStatic(graph->g, *params, static_cast<hipStream_t>(stream), labels_out, stats, trace_out,
+                              started);
     stats->seconds = cc::hip::Since(started);
   });
 }

@@ -61,8 +61,22 @@ class DevicePool {
   size_t reserved_ = 0, in_use_ = 0;
 };
 
+// A fixed device region handed out front to back in 256-byte units: the static scratch region of the solve that
+// allocates nothing. Blocks are not returned; the next call starts over at the region's start.
+class ScratchArena {
+ public:
+  ScratchArena(void* base, size_t bytes) : base_(static_cast<uint8_t*>(base)), bytes_(bytes) {}
+  void* Take(size_t bytes);  // CC_HIP_ENOMEM beyond the region
+  size_t used() const { return used_; }
+
+ private:
+  uint8_t* base_;
+  size_t bytes_;
+  size_t used_ = 0;
+};
+
 // Device allocations released together: by hipFree, or back to a DevicePool when one is set before the first
-// allocation.
+// allocation; blocks taken from a ScratchArena are not released at all.
 class DeviceMemory {
  public:
   DeviceMemory() = default;
@@ -70,10 +84,12 @@ class DeviceMemory {
   DeviceMemory& operator=(const DeviceMemory&) = delete;
   ~DeviceMemory();
   void UsePool(DevicePool* pool) { pool_ = pool; }
+  void UseArena(ScratchArena* arena) { arena_ = arena; }
   void* AllocateBytes(size_t bytes);
 
  private:
   DevicePool* pool_ = nullptr;
+  ScratchArena* arena_ = nullptr;
   std::vector<void*> blocks_;
 };
 
@@ -175,7 +191,8 @@ struct DescentGroup {
 // sets it takes part in, so a serving lane can add and drop runs between iterations.
 class RunState {
  public:
-  RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool = nullptr);
+  // Memory from hipMalloc, or from `pool`, or from `arena` (then nothing is allocated).
+  RunState(const RunSpec& spec, hipStream_t stream, DevicePool* pool = nullptr, ScratchArena* arena = nullptr);
   RunState(const RunState&) = delete;
   RunState& operator=(const RunState&) = delete;
   const RunSpec& spec() const { return spec_; }
@@ -194,11 +211,14 @@ class RunState {
 // The launches over a set of runs: the device run table, the item tables of every launch (slots, rebuild
 // tiles, descent items grouped by kernel instance and ordered largest run first) and the persistent-queue
 // counters. An iteration of all runs is one fixed launch sequence. With a pool and a pinned buffer (both kept by the
-// caller), the set allocates and frees nothing through the HIP runtime.
+// caller), the set allocates and frees nothing through the HIP runtime; with an arena it allocates no device or
+// pinned memory at all (the control blocks come back into pageable host memory).
 class LaunchSet {
  public:
   LaunchSet(const std::vector<RunState*>& states, hipStream_t stream, DevicePool* pool = nullptr,
-            PinnedBuffer* pinned = nullptr);
+            PinnedBuffer* pinned = nullptr, ScratchArena* arena = nullptr);
+  // Device bytes of the tables a set over runs of these specs takes (what it draws from a pool or an arena).
+  static size_t TableBytes(const std::vector<RunSpec>& specs);
   ~LaunchSet();
   LaunchSet(const LaunchSet&) = delete;
   LaunchSet& operator=(const LaunchSet&) = delete;
@@ -236,12 +256,14 @@ class LaunchSet {
   int max_n_ = 0;
   Ctl* host_ctl_ = nullptr;
   bool own_host_ctl_ = false;
+  std::vector<Ctl> pageable_ctl_;  // the control blocks' host copy when no pinned memory may be allocated
 };
 
 // A fixed set of runs with its launch set; iterations are replayed from one captured HIP graph.
 class Batch {
  public:
-  Batch(const std::vector<RunSpec>& specs, hipStream_t stream);
+  // With an arena every piece of device state comes from it, and Iterate must be called with use_graph false.
+  Batch(const std::vector<RunSpec>& specs, hipStream_t stream, ScratchArena* arena = nullptr);
   ~Batch();
   Batch(const Batch&) = delete;
   Batch& operator=(const Batch&) = delete;
@@ -274,6 +296,10 @@ void RunToCompletion(Batch& batch, const std::vector<RunSpec>& specs, bool verbo
 
 void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_out, cc_pbils_stats* stats,
                 int64_t* trace_out, Clock::time_point started);
+// SolvePbils with all device state in the static scratch region of the current device (no allocation, no HIP graph,
+// no event timing), on `stream`.
+void SolvePbilsStatic(const DeviceGraph& g, const cc_pbils_params& p, hipStream_t stream, int32_t* labels_out,
+                      cc_pbils_stats* stats, int64_t* trace_out, Clock::time_point started);
 void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool local_search, int32_t* labels_out,
                        cc_pbils_stats* stats, Clock::time_point started);
 

@@ -124,6 +124,21 @@ int cc_hip_graph_solve_pbils(cc_hip_graph* graph, const cc_pbils_params* params,
 int cc_hip_graph_solve_neighborhood(cc_hip_graph* graph, const cc_pbils_params* params, int32_t local_search,
                                     int32_t* labels_out, cc_pbils_stats* stats, char* err, size_t err_len);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): cc_hip_graph_solve_pbils with every piece of device state
+ * (the two populations, G, record, trace, GWW and rebuild scratch, launch tables) laid out in a static region of
+ * CC_HIP_STATIC_SCRATCH_BYTES per GPU, held from the library's load until the process ends, instead of fresh
+ * allocations. No hipMalloc, hipHostMalloc or HIP graph instantiation: iterations are launched one by one on
+ * `stream` (a hipStream_t; NULL: the default stream) and params.profile must be 0 (CC_HIP_EINVAL otherwise). A run
+ * whose state exceeds the region (cc_hip_footprint minus the graph, plus the launch tables) fails with
+ * CC_HIP_ENOMEM before anything runs. Every call on one GPU uses the same region and leaves its contents behind:
+ * calls that overlap on one GPU (other host threads, other streams) race and must not be made. Results are
+ * bit-identical to cc_hip_graph_solve_pbils. */
+#define CC_HIP_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int cc_hip_graph_solve_pbils_static(cc_hip_graph* graph, const cc_pbils_params* params, void* stream,
+                                    int32_t* labels_out, cc_pbils_stats* stats, int64_t* trace_out, char* err,
+                                    size_t err_len);
+
 /* ---- batched solves: many independent PBILS runs (different graphs, n, k and parameters) advanced together by
  *      one launch per phase, descents claimed from persistent work queues, iterations replayed from a HIP graph.
  *      Each run's result is bit-identical to its own cc_hip_graph_solve_pbils. No reference counterpart: the
