@@ -5,14 +5,17 @@
 A step is one PBILS iteration (select + steepest descent of every individual + record + GWW + perturb + G
 rebuild) of the planted-partition workload: SBM n=10000 in 5 equal hidden clusters, p_in 0.4, p_out 0.05,
 graph-seed 2; k=5, pop 128, gww 0.1, seed 42, early stop off. The graph is uploaded to HBM before the timed region;
-the timed region is one solve of K iterations (its allocation and initial rebuild included). Prints one JSON line.
+the timed region is one solve of K iterations (its allocation and initial rebuild included), with the iterations
+replayed from the library's captured HIP graph as any caller's solve is. Prints one JSON line.
 
 value       PBILS iterations per second (whole job, 1 GPU)
-roofline    the descent kernel (the dominant one): algorithmic global-memory bytes per launch / its mean launch
-            time, measured with HIP events on the library's stream over the timed region; for the register kernels
-            also the VALU figure (algorithmic lane operations per move), the roof that binds them
-cpu_baseline  the reference CPU solver (built out of tree into oracle/_ref) on this host's cores, on a bounded
-            sample of the same workload; bit_exact reports that the GPU reproduces its result on that sample.
+roofline    the descent kernel (95 % of the device time), its launch time taken with HIP events on the library's
+            stream in a second solve of the same trajectory: VALU issue from the instruction counts per move that
+            rocprofv3 measured for this plan (MEASURED; profiles/), with global-memory bytes and the algorithmic
+            lane operations beside it
+cpu_baseline  the reference CPU solver (built out of tree into oracle/_ref) on this host's cores: the median time
+            per iteration over three pairs of solves after a warm-up; bit_exact reports that the GPU reproduces its
+            result on those solves.
 """
 import argparse
 import json
@@ -30,7 +33,12 @@ sys.path.insert(0, ROOT)
 # them. valu: wave64 VALU instructions (SQ_INSTS_VALU); lds: LDS instructions (SQ_INSTS_LDS); lds_conflict: cycles
 # of LDS bank conflicts (SQ_LDS_BANK_CONFLICT); fabric: bytes moved beyond L2, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (KiB -> bytes; the factors from a 1 GiB device copy, profiles/copy_calibration_1gib.txt).
-MEASURED = {}
+MEASURED = {
+    ("c2", "registers", 1024, 10): dict(valu=2192.2, lds=224.0, lds_conflict=0.0, fabric=869.6,
+                                        profile="profiles/descent_registers_sbm_n10000_k5.txt"),
+    ("c5", "k3-chunk", 1024, 0): dict(valu=8539.0, lds=301.2, lds_conflict=29.0, fabric=29824.0,
+                                      profile="profiles/descent_k3chunk_chunglu_n100000_k3.txt"),
+}
 
 METRIC = "PBILS iter/s + LS moves/s (1 GPU); batched graphs/s & p99 vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
