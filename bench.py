@@ -503,14 +503,80 @@ def bench_sweep(cga, args):
                       "bit_exact_vs_reference_f": exact, "points": out_points}))
 
 
+# The reference's published wide-path runs on its two real tag graphs (docs/benchmarks.md:306-331; RTX 4070 Ti SUPER,
+# pop 128, tournament 5, perturbation 0.4, early stop 6, --seed 5): (n, edge density, k) -> (seconds per iteration
+# after the first, first iteration, accepted moves per descent).
+REFERENCE_WIDE = {
+    (193734, 0.126, 2): (10.2, 14.9, 81451),
+    (193734, 0.126, 3): (23.3, 41.7, 87592),
+    (330000, 0.128, 2): (41.6, 55.1, 137757),
+    (330000, 0.128, 3): (105.0, 181.9, 145887),
+}
+
+
+def bench_scale(cga, args):
+    """The wide path at the reference's published scale: synthetic ER graphs with the vertex counts and edge densities
+    of its two tag graphs (NUS-WIDE, all 193 734 objects, 12.6 %; Stack Overflow sample, 330 000 objects, 12.8 %; the
+    graphs themselves need downloaded data), k = 2 and 3, pop 128 and the reference's protocol (tournament 5,
+    perturbation 0.4, gww 0.1, seed 5). Graph resident in HBM; the first iteration (descents from random labels) is
+    timed by a solve of one iteration, the later ones by the difference to a solve of three. Each result's f must
+    equal the library's own recount of its labels. The per-move figure divides an iteration by the accepted moves of
+    one descent (all slots descend at once), which compares graphs whose descents differ in length. One JSON line."""
+    sizes = [(int(a), float(b)) for a, b in (s.split(":") for s in args.scale_sizes.split(","))]
+    common = dict(population=128, tournament=5, perturbation=0.4, gww=0.1, seed=5)
+    points = []
+    for n, density in sizes:
+        t0 = time.perf_counter()
+        g = cga.erdos_renyi(n, density, 7)
+        note(f"scale: ER n={n} p={density} built in {time.perf_counter() - t0:.1f} s ({g.edges} edges)")
+        with cga.DeviceGraph(g) as dg:
+            for k in (2, 3):
+                runs = {}
+                for iters in (1, 3):
+                    t0 = time.perf_counter()
+                    res = dg.solve_pbils(k=k, iterations=iters, early_stop=1000, **common)
+                    runs[iters] = (time.perf_counter() - t0, res)
+                    note(f"scale n={n} k={k}: {iters} iteration(s) in {runs[iters][0]:.2f} s")
+                    if cga.objective_bitwise(g, res["labels"]) != res["objective"]:
+                        raise SystemExit(f"scale n={n} k={k}: reported f differs from the recount")
+                first_s, first = runs[1]
+                later_s = (runs[3][0] - first_s) / 2
+                moves = (runs[3][1]["accepted_moves"] - first["accepted_moves"]) / (2 * common["population"])
+                # all slots descend at once: algorithmic bytes of one move of every slot per move time, against HBM
+                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n)
+                gbs = common["population"] * per_move / (later_s / moves) / 1e9
+                point = {"n": n, "density": density, "edges": g.edges, "k": k, "descent": first["descent_name"],
+                         "iteration_s": round(later_s, 3), "first_iteration_s": round(first_s, 3),
+                         "moves_per_descent": round(moves), "us_per_move": round(1e6 * later_s / moves, 2),
+                         "hbm": {"bytes_per_move": round(per_move), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 3)},
+                         "f_after_3": runs[3][1]["objective"], "f_recounted": True}
+                ref = REFERENCE_WIDE.get((n, density, k))
+                if ref:
+                    point["reference"] = {"iteration_s": ref[0], "first_iteration_s": ref[1],
+                                          "moves_per_descent": ref[2],
+                                          "us_per_move": round(1e6 * ref[0] / ref[2], 2)}
+                    point["per_move_vs_reference"] = round(point["reference"]["us_per_move"] / point["us_per_move"], 1)
+                points.append(point)
+                note(f"scale n={n} k={k} {point['descent']}: iteration {later_s:.2f} s, first {first_s:.2f} s, "
+                     f"{moves:.0f} moves per descent")
+    print(json.dumps({"metric": "wide-path PBILS iteration at the reference's published graph sizes",
+                      "unit": "s", "higher_is_better": False, "data": "synthetic",
+                      "config": {"workload": "ER graphs with the reference's tag-graph sizes and densities; k 2 and 3, "
+                                             "pop 128, tournament 5, perturbation 0.4, gww 0.1, seed 5"},
+                      "points": points}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "tags", "sweep"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "tags", "sweep", "scale"])
     ap.add_argument("--sweep-max-n", type=int, default=32000, help="largest n of --config sweep")
     ap.add_argument("--tags-objects", type=int, default=60000, help="objects of the --config tags file")
+    ap.add_argument("--scale-sizes", default="193734:0.126,330000:0.128",
+                    help="--config scale: comma-separated n:density pairs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
     ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
@@ -539,6 +605,9 @@ def main():
         return
     if args.config == "sweep":
         bench_sweep(cga, args)
+        return
+    if args.config == "scale":
+        bench_scale(cga, args)
         return
     if args.config == "c3":
         bench_neighborhood(cga, args)

@@ -860,62 +860,121 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
   }
 }
 
-// Vertices per thread whose loads a wide-path scan keeps in flight per group.
-constexpr int kWideUnroll = 8;
-
 template <typename Word>
 using GlobalPtr = __attribute__((address_space(1))) Word*;
+// 16 bytes of vertex words, one global_load_dwordx4 / global_store_dwordx4 per lane.
+using Quad = uint32_t __attribute__((ext_vector_type(4)));
 
-// One pass of a wide-path descent over a slot's packed vertex words after a move: the word of every neighbour of
-// the moved vertex gets `inc` and is written back, then visit(w, u) sees every word u < n of the thread's share.
-// Words are loaded in groups of kWideUnroll per thread and group g + 1 is loaded before group g is consumed, so two
-// groups of loads are in flight behind the arithmetic. The row of the moved vertex is read as one 32-bit half word
-// per lane (lanes 0-31 the low half of the 64-bit word, 32-63 the high half); a group past n reads, clamped, at
-// most one half word beyond the row (the next row or the zero tail after the last row) and discards it.
-template <int T, typename Word, typename Visit>
-__device__ __forceinline__ void StreamMove(GlobalPtr<Word> st, GlobalWords32 row, int n, int wave, int lane, Word inc,
-                                           Visit&& visit) {
-  constexpr int kU = kWideUnroll;
-  const int half = lane >> 5;
-  const int bitpos = lane & 31;
-  Word cw[kU], nw[kU];
-  uint32_t cr[kU], nr[kU];
-  auto load = [&](int base0, Word* w, uint32_t* rr) {
+// One pass of a streaming wide-path descent over a slot's packed vertex words (Word: uint32_t at k = 2, uint64_t at
+// k = 3) after a move, 16 bytes per lane. With kV = 16 / sizeof(Word), a wave's tile is kStreamLoads loads: lane l of
+// load j holds the kV consecutive vertices u = tile * kTile + (64 j + l) kV + i, whose row bits are bits
+// (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tile t belongs to wave t % kWaves (the
+// same owner as in ForEachOwned), and a wave loads its next tile before it consumes the current one, so 2 x 64 B of
+// words and two row dwords per lane are in flight behind the arithmetic. Every word of a neighbour of the moved
+// vertex gets `inc`, the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of itself), a
+// 16-byte group with a changed word is written back, and visit(w, u) sees every vertex u < n in increasing order
+// within the thread. `st` is the slot's base (16-byte aligned, padded to whole tiles: the padding
+// words take increments and are written back, never visited). Row dwords of the last tile may lie beyond the row
+// (the next row, or the zero tail after the last row); their bits reach only padding words.
+template <int T, typename Word, typename Relabel, typename Visit>
+__device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, GlobalWords32 row, int n, int wave, int lane, Word inc,
+                                           int mover, Relabel&& relabel, Visit&& visit) {
+  constexpr int kV = 16 / static_cast<int>(sizeof(Word));
+  constexpr int kJ = kStreamLoads;
+  constexpr int kTile = StreamTile(sizeof(Word));
+  constexpr int kWaves = T / kWave;
+  const int tiles = (n + kTile - 1) / kTile;
+  const uint32_t lane_bit = static_cast<uint32_t>(lane * kV) & 31u;
+  const int lane_dword = (lane * kV) >> 5;
+  // the moved vertex: its tile and load (uniform), lane and word
+  const int mover_tile = mover / kTile;
+  const int mover_load = (mover - mover_tile * kTile) / (kWave * kV);
+  const int mover_lane = (mover / kV) % kWave;
+  const int mover_word = mover % kV;
+  Quad cw[kJ], nw[kJ];
+  uint32_t cr[kJ], nr[kJ];
+  auto load = [&](int t, Quad* w, uint32_t* rr) {
 #pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      const int base = base0 + i * T;  // wave-uniform
-      const int u = base + lane;
-      rr[i] = row[((base < n ? base : n - 1) >> 5) + half];
-      w[i] = u < n ? st[u] : Word(0);
+    for (int j = 0; j < kJ; ++j) {
+      w[j] = st[static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane];
+      rr[j] = row[((t * kTile + j * kWave * kV) >> 5) + lane_dword];
     }
   };
-  load(wave * kWave, cw, cr);
-  for (int base0 = wave * kWave; base0 < n; base0 += T * kU) {
-    load(base0 + T * kU, nw, nr);
+  auto consume = [&](int t, auto full) {
 #pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      const int u = base0 + i * T + lane;
-      if (u < n) {
-        Word w = cw[i];
-        if ((cr[i] >> bitpos) & 1u) {
-          w += inc;
-          st[u] = w;
+    for (int j = 0; j < kJ; ++j) {
+      const uint32_t bits = cr[j] >> lane_bit;
+      Word ws[kV];
+      __builtin_memcpy(ws, &cw[j], 16);
+#pragma unroll
+      for (int i = 0; i < kV; ++i) ws[i] += inc & (Word(0) - static_cast<Word>((bits >> i) & 1u));
+      bool changed = (bits & ((1u << kV) - 1u)) != 0;
+      if (t == mover_tile && j == mover_load && lane == mover_lane) {  // once per move, in one lane
+#pragma unroll
+        for (int i = 0; i < kV; ++i) {
+          if (i == mover_word) ws[i] = relabel(ws[i]);
         }
-        visit(w, u);
+        changed = true;
+      }
+      if (changed) {
+        Quad out;
+        __builtin_memcpy(&out, ws, 16);
+        st[static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane] = out;
+      }
+      const int u0 = t * kTile + (j * kWave + lane) * kV;
+#pragma unroll
+      for (int i = 0; i < kV; ++i) {
+        if (decltype(full)::value || u0 + i < n) visit(ws[i], u0 + i);
       }
     }
+  };
+  int t = wave;
+  if (t < tiles) load(t, cw, cr);
+  for (; t < tiles; t += kWaves) {
+    if (t + kWaves < tiles) load(t + kWaves, nw, nr);
+    if ((t + 1) * kTile <= n) {  // wave-uniform
+      consume(t, std::true_type{});
+    } else {
+      consume(t, std::false_type{});
+    }
 #pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      cw[i] = nw[i];
-      cr[i] = nr[i];
+    for (int j = 0; j < kJ; ++j) {
+      cw[j] = nw[j];
+      cr[j] = nr[j];
     }
   }
 }
 
-// Wide path, k = 2: one 32-bit word per vertex, w = 2 e + label with e = G1 - G0, kept in the slot's row G[0]
-// during the descent (reference algebra cuda/cc_local_search_k2.cuh:251-254): delta = sv - w for label 0 and
-// w + 1 - sv for label 1; a neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored
-// from w and the degrees at the end.
+// The vertices u < n of a slot's streaming state that this thread owns (StreamMove's layout), in increasing order:
+// visit(u).
+template <int T, int kWordBytes, typename Visit>
+__device__ __forceinline__ void ForEachOwned(int n, int wave, int lane, Visit&& visit) {
+  constexpr int kV = 16 / kWordBytes;
+  constexpr int kTile = StreamTile(kWordBytes);
+  for (int t = wave; t * kTile < n; t += T / kWave) {
+#pragma unroll
+    for (int j = 0; j < kStreamLoads; ++j) {
+      const int u0 = t * kTile + (j * kWave + lane) * kV;
+#pragma unroll
+      for (int i = 0; i < kV; ++i) {
+        if (u0 + i < n) visit(u0 + i);
+      }
+    }
+  }
+}
+
+// The thread that owns vertex v in StreamMove's layout.
+template <int T, int kWordBytes>
+__device__ __forceinline__ int StreamOwner(int v) {
+  constexpr int kTile = StreamTile(kWordBytes);
+  const int t = v / kTile;
+  return (t % (T / kWave)) * kWave + ((v - t * kTile) / (16 / kWordBytes)) % kWave;
+}
+
+// Wide path, k = 2: one 32-bit word per vertex, w = 2 e + label with e = G1 - G0, in the run's work buffer
+// (reference algebra cuda/cc_local_search_k2.cuh:251-254): delta = sv - w for label 0 and w + 1 - sv for label 1; a
+// neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored from w and the degrees at the
+// end.
 template <int T>
 __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
@@ -933,26 +992,26 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   const int src = src_sh;
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 2 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
-  int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
-  int32_t* state = g_out;  // row G[0] of the target slot
+  uint32_t* state = reinterpret_cast<uint32_t*>(r.work) + static_cast<size_t>(s) * WideStride(n, 4);
   int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
   const int64_t f_start = r.pop.f[src];
 
   int sv = n - 2 * n0 + 1;
   int best_d = 0, best_u = -1, best_w = 0;
-  for (int base = wave * kWave; base < n; base += T) {
-    const int u = base + lane;
-    if (u < n) {
-      const int w = 2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1);
-      state[u] = w;
-      const int d = (w & 1) ? w + 1 - sv : sv - w;
-      if (d < best_d) {
-        best_d = d;
-        best_u = u;
-        best_w = w;
-      }
+  auto consider = [&](uint32_t word, int u) {
+    const int w = static_cast<int>(word);
+    const int d = (w & 1) ? w + 1 - sv : sv - w;
+    if (d < best_d) {
+      best_d = d;
+      best_u = u;
+      best_w = w;
     }
-  }
+  };
+  ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
+    const uint32_t w = static_cast<uint32_t>(2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1));
+    state[u] = w;
+    consider(w, u);
+  });
   uint64_t key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
   int64_t objective = f_start;
   int64_t applied = 0;
@@ -961,7 +1020,8 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
-  const GlobalPtr<int32_t> st = (GlobalPtr<int32_t>)state;
+  const GlobalPtr<Quad> st = (GlobalPtr<Quad>)state;
+  bool capped = false;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -972,38 +1032,32 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
     const int from = static_cast<int>((mv >> 6) & 63u);
     n0 += from ? 1 : -1;
     sv = n - 2 * n0 + 1;
-    const int shift = from ? -4 : 4;
-    if (v % T == tid) st[v] ^= 1;
+    const uint32_t inc = from ? static_cast<uint32_t>(-4) : 4u;
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_u = -1;
-    StreamMove<T, int32_t>(st, row, n, wave, lane, shift, [&](int w, int u) {
-      const int d = (w & 1) ? w + 1 - sv : sv - w;
-      if (d < best_d) {
-        best_d = d;
-        best_u = u;
-        best_w = w;
-      }
-    });
+    StreamMove<T, uint32_t>(st, row, n, wave, lane, inc, v, [](uint32_t w) { return w ^ 1u; }, consider);
     key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
     objective += delta;
     if (++applied >= cap) {
-      if (tid == 0) {
-        ctl->error = 1;
-        ctl->stopped = 1;
-      }
+      capped = true;
       break;
     }
   }
+  if (capped && tid == 0) {
+    ctl->error = 1;
+    ctl->stopped = 1;
+  }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
-  for (int u = tid; u < n; u += T) {
-    const int w = state[u];
+  int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
+  ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
+    const int w = static_cast<int>(state[u]);
     const int e = w >> 1;
     const int degree = r.deg[u];
     lab_out[u] = static_cast<uint8_t>(w & 1);
     g_out[u] = (degree - e) / 2;
     g_out[n + u] = (degree + e) / 2;
-  }
+  });
   if (tid == 0) {
     r.next.sizes[static_cast<size_t>(s) * 2] = n0;
     r.next.sizes[static_cast<size_t>(s) * 2 + 1] = n - n0;
@@ -1024,7 +1078,6 @@ constexpr int kK3LabelShift = 62;
 template <int T>
 __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   Ctl* ctl = r.ctl;
-  uint64_t* work = r.work;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
   const int tid = threadIdx.x;
@@ -1040,7 +1093,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
-  uint64_t* state = work + static_cast<size_t>(s) * n;
+  uint64_t* state = r.work + static_cast<size_t>(s) * WideStride(n, 8);
   int s0 = sizes_in[0], s1 = sizes_in[1], s2 = sizes_in[2];
   const int64_t f_start = r.pop.f[src];
 
@@ -1064,16 +1117,13 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   auto key_of = [&]() {
     return best_move < 0 ? kNoMove : PackMove(best_d, best_move >> 4, (best_move >> 2) & 3, best_move & 3);
   };
-  for (int base = wave * kWave; base < n; base += T) {
-    const int u = base + lane;
-    if (u < n) {
-      const uint64_t w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
-                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
-                         static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
-      state[u] = w;
-      consider(w, u);
-    }
-  }
+  ForEachOwned<T, 8>(n, wave, lane, [&](int u) {
+    const uint64_t w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
+                       static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
+                       static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
+    state[u] = w;
+    consider(w, u);
+  });
   uint64_t key = key_of();
   int64_t objective = f_start;
   int64_t applied = 0;
@@ -1082,7 +1132,8 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
-  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)state;
+  const GlobalPtr<Quad> st = (GlobalPtr<Quad>)state;
+  bool capped = false;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
     parity ^= 1;
@@ -1096,32 +1147,33 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
     s1 += (to == 1) - (from == 1);
     s2 += (to == 2) - (from == 2);
     const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
-    if (v % T == tid) {
-      st[v] = (st[v] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
-    }
+    const uint64_t label_to = static_cast<uint64_t>(to) << kK3LabelShift;
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_move = -1;
-    StreamMove<T, uint64_t>(st, row, n, wave, lane, inc, consider);
+    StreamMove<T, uint64_t>(
+        st, row, n, wave, lane, inc, v,
+        [label_to](uint64_t w) { return (w & ~(3ull << kK3LabelShift)) | label_to; }, consider);
     key = key_of();
     objective += delta;
     if (++applied >= cap) {
-      if (tid == 0) {
-        ctl->error = 1;
-        ctl->stopped = 1;
-      }
+      capped = true;
       break;
     }
   }
+  if (capped && tid == 0) {
+    ctl->error = 1;
+    ctl->stopped = 1;
+  }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
-  for (int u = tid; u < n; u += T) {
+  ForEachOwned<T, 8>(n, wave, lane, [&](int u) {
     const uint64_t w = state[u];
     g_out[u] = static_cast<int32_t>(w & kK3Mask);
     g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
     g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
     lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
-  }
+  });
   if (tid == 0) {
     int32_t* sizes = r.next.sizes + static_cast<size_t>(s) * 3;
     sizes[0] = s0;

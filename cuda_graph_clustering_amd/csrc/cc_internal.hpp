@@ -63,7 +63,8 @@ struct RunDev {
   int32_t* order;        // [P] GWW ranking scratch
   uint64_t* masks;       // [P][k][words] rebuild scratch
   int64_t* intra;        // [P] rebuild scratch: sum_v G[label v][v]
-  uint64_t* work;        // [P][n] packed vertex words of the k = 3 wide descent (null for other plans)
+  uint64_t* work;        // packed vertex words of the wide descents (null for other plans): k = 3 chunk kernel
+                         // [P][n] u64; streaming kernels [P][WideStride(n, word)] of u32 (k = 2) or u64 (k = 3)
   uint64_t init_stream, select_stream, perturb_stream;
   int32_t tournament, copies, early_stop, iterations;
   float perturbation;
@@ -91,6 +92,15 @@ struct ItemList {
 // therefore only be reached through a broken invariant (a wrong delta or G): the descent then stops, sets
 // ctl->error, and the run fails instead of spinning on the device. No valid descent is cut short.
 __host__ __device__ inline int64_t MoveCap(int64_t f_start) { return f_start + 1; }
+
+// The streaming wide-path descents (k = 2: 4-byte vertex words, k = 3: 8-byte words) read and write a slot's words
+// 16 bytes per lane. A wave's tile is kStreamLoads such loads of its 64 lanes; each slot's words start 16-byte
+// aligned and are padded to whole tiles (WideStride words per slot in RunDev::work).
+constexpr int kStreamLoads = 4;
+__host__ __device__ constexpr int StreamTile(int word_bytes) { return 64 * (16 / word_bytes) * kStreamLoads; }
+__host__ __device__ constexpr int64_t WideStride(int n, int word_bytes) {
+  return (static_cast<int64_t>(n) + StreamTile(word_bytes) - 1) / StreamTile(word_bytes) * StreamTile(word_bytes);
+}
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 

@@ -233,14 +233,29 @@ int GenericThreads(int n) {
   return 1024;
 }
 
+// Whether the k = 3 chunk-summary descent suits the graph: a move recomputes the 64-vertex chunks that hold the
+// moved vertex's neighbours, about min(mean degree, chunks) of them, while the streaming kernel passes over every
+// vertex word. Measured at n = 40000 (625 chunks, ER, pop 64, 2 iterations): chunk / streaming descent time 0.26 at
+// mean degree 16, 0.38 at 64, 0.91 at 256, and with the chunk kernel 2.2 and 3.5 times the streaming kernel's at
+// 1024 and 4096. The chunk kernel is taken while twice the mean degree is at most the chunk count; with no edge count
+// (a plan from the shape alone) the graph is taken as sparse.
+bool SparseForChunks(uint32_t n, uint64_t edges) {
+  const double mean_degree = 2.0 * static_cast<double>(edges) / std::max<double>(n, 1);
+  return 2.0 * mean_degree <= static_cast<double>(Words64(n));
+}
+
 }  // namespace
 
 // The descent kernel and its launch shape (reference PlanDescent, cuda/cc_pbils_gpu.cu:69-102).
-DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
+DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, uint64_t edges) {
   const int n = static_cast<int>(n_);
   const int wide = WidePath(n_, p) ? 1 : 0;
   DescentPlan plan;
-  if (p.ls_kernel == CC_LS_AUTO && TunedPlan(n_, k, p.population, wide != 0, &plan)) return plan;
+  // A tuned plan serves `auto`, and `wide` when it is a wide-path plan (the tuner's choice among the wide kernels).
+  if ((p.ls_kernel == CC_LS_AUTO || (p.ls_kernel == CC_LS_WIDE && wide)) &&
+      TunedPlan(n_, k, p.population, wide != 0, &plan)) {
+    return plan;
+  }
   plan.k = k;
   plan.wide = wide;
   plan.threads = GenericThreads(n);
@@ -250,11 +265,12 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p) {
     Fail(CC_HIP_EINVAL, "state needs %zu bytes of LDS, the device allows %zu", lds, DescentLdsLimit());
   }
   // Wide path: the packed k = 2 / k = 3 kernels unless the generic kernel is asked for
-  // (reference cuda/cc_pbils_gpu.cu:84-96); for k = 3 the chunk-summary variant while its table fits in LDS.
+  // (reference cuda/cc_pbils_gpu.cu:84-96); for k = 3 the chunk-summary variant while its table fits in LDS and the
+  // graph is sparse enough that a move leaves most 64-vertex chunks untouched (SparseForChunks).
   if (wide && p.ls_kernel != CC_LS_GLOBAL && (k == 2 || k == 3)) {
     plan.descent = k == 2 ? Descent::kK2Wide : Descent::kK3Wide;
     plan.threads = n >= 8192 ? 1024 : 256;
-    if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit()) {
+    if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit() && SparseForChunks(n_, edges)) {
       plan.descent = Descent::kK3Chunk;
       plan.lds_bytes = K3ChunkLdsBytes(n);
     }
@@ -404,8 +420,12 @@ void LayoutRun(const RunSpec& s, RunDev& r, Arena& a) {
   a.Take(r.order, P);
   a.Take(r.masks, P * k * s.graph->words);
   a.Take(r.intra, P);
-  const bool packed3 = s.plan.descent == Descent::kK3Wide || s.plan.descent == Descent::kK3Chunk;
-  a.Take(r.work, packed3 ? P * n : 0);
+  size_t work = 0;  // in 64-bit words; the same for both k = 3 kernels, so a footprint needs no edge count
+  if (s.plan.descent == Descent::kK3Wide || s.plan.descent == Descent::kK3Chunk) {
+    work = P * static_cast<size_t>(WideStride(static_cast<int>(n), 8));
+  }
+  if (s.plan.descent == Descent::kK2Wide) work = P * static_cast<size_t>(WideStride(static_cast<int>(n), 4)) / 2;
+  a.Take(r.work, work);
 }
 
 }  // namespace
@@ -795,7 +815,7 @@ void RunToCompletion(Batch& batch, const std::vector<RunSpec>& specs, bool verbo
 void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_out, cc_pbils_stats* stats,
                 int64_t* trace_out, Clock::time_point started) {
   CheckParams(p);
-  RunSpec spec{&g, p, PlanDescent(g.n, p.k, p), p.population, true};
+  RunSpec spec{&g, p, PlanDescent(g.n, p.k, p, g.edges), p.population, true};
   Stream stream;
   Batch batch({spec}, stream.get());
   if (p.verbose) PrintPlan(batch.Plan(0), batch.Bytes());
@@ -850,7 +870,7 @@ void SolvePbilsStatic(const DeviceGraph& g, const cc_pbils_params& p, hipStream_
                       cc_pbils_stats* stats, int64_t* trace_out, Clock::time_point started) {
   CheckParams(p);
   if (p.profile) Fail(CC_HIP_EINVAL, "the solve without allocation takes no event timings (params.profile)");
-  const RunSpec spec{&g, p, PlanDescent(g.n, p.k, p), p.population, true};
+  const RunSpec spec{&g, p, PlanDescent(g.n, p.k, p, g.edges), p.population, true};
   const size_t need = RunState::Footprint(spec) + LaunchSet::TableBytes({spec});
   if (need > kStaticScratchBytes) {
     Fail(CC_HIP_ENOMEM, "the run needs %s of device state, more than the %s static scratch region", Mebibytes(need).c_str(),
@@ -888,7 +908,7 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   p.gww = 0.0;
   p.iterations = 1;
   p.early_stop = 1;
-  RunSpec spec{&g, p, PlanDescent(g.n, 2, p), p.population, false};
+  RunSpec spec{&g, p, PlanDescent(g.n, 2, p, g.edges), p.population, false};
   Stream stream;
   const hipStream_t s = stream.get();
   Batch batch({spec}, s);
@@ -974,7 +994,8 @@ void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
   std::vector<RunSpec> specs;
   for (BatchRun& b : runs) {
     CheckParams(b.params);
-    RunSpec spec{b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params), b.params.population, true};
+    RunSpec spec{b.graph, b.params, PlanDescent(b.graph->n, b.params.k, b.params, b.graph->edges),
+                 b.params.population, true};
     if (runs.size() > 1) UseMixedDescent(spec);
     specs.push_back(spec);
   }

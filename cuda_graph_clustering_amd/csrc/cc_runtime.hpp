@@ -124,7 +124,9 @@ void Upload(DeviceGraph& g, const uint64_t* bits, uint32_t n, uint64_t edges, hi
             DevicePool* pool = nullptr);
 void CheckParams(const cc_pbils_params& p);
 bool WidePath(uint32_t n, const cc_pbils_params& p);
-DescentPlan PlanDescent(uint32_t n, int k, const cc_pbils_params& p);
+// `edges` (of the graph) decides between the two k = 3 wide kernels; 0 for a plan made from the shape alone, whose
+// memory footprint does not depend on that choice.
+DescentPlan PlanDescent(uint32_t n, int k, const cc_pbils_params& p, uint64_t edges);
 // The autotuner's recorded plan for a shape, if any (cc_tune.hip).
 bool TunedPlan(uint32_t n, int k, int population, bool wide, DescentPlan* plan);
 

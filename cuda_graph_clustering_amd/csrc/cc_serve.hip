@@ -138,7 +138,7 @@ class Server {
     DeviceGraph shape;
     shape.n = n;
     shape.words = Words64(n);
-    const RunSpec spec{&shape, params, PlanDescent(n, params.k, params), params.population, true};
+    const RunSpec spec{&shape, params, PlanDescent(n, params.k, params, 0), params.population, true};
     return GraphBytes(n) + RunState::Footprint(spec);
   }
 
@@ -211,7 +211,7 @@ class Server {
           Upload(*a.graph, req.bits.data(), req.n, req.edges, s, &pool_);
           req.bits.clear();
           req.bits.shrink_to_fit();
-          RunSpec spec{a.graph.get(), req.params, PlanDescent(req.n, req.params.k, req.params),
+          RunSpec spec{a.graph.get(), req.params, PlanDescent(req.n, req.params.k, req.params, req.edges),
                        req.params.population, true};
           UseMixedDescent(spec);  // the lane's register runs share the mixed launches
           a.state = std::make_unique<RunState>(spec, s, &pool_);

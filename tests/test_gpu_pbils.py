@@ -279,6 +279,63 @@ def test_device_graph_repeated_solves_are_identical(cga):
     assert (a["labels"] == b["labels"]).all() and (a["labels"] == c["labels"]).all()
 
 
+def run_with_plan(cga, orc, g, cpu_threads, tmp_path, plan, **kw):
+    """Solve with one tuned-plan entry forced (the table line: n k pop wide descent threads vpt rows_in_lds lds
+    blocks) and with the oracle."""
+    table = tmp_path / "plan.txt"
+    table.write_text(plan + "\n")
+    cga.tune_clear()
+    try:
+        assert cga.tune_load(table) == 1
+        return run_both(cga, orc, g, cpu_threads, **kw)
+    finally:
+        cga.tune_clear()
+
+
+@pytest.fixture(scope="module")
+def dense_wide_graph(cga):
+    """ER n=33001, p=0.12: a dense graph on the wide path (mean degree about 3960, every 64-vertex chunk touched by
+    every move), n not a multiple of the streaming kernels' tiles (1024 vertices at k = 2, 512 at k = 3)."""
+    return cga.erdos_renyi(33001, 0.12, 21)
+
+
+@pytest.mark.parametrize("k,threads,blocks", [(2, 1024, 0), (3, 1024, 0), (2, 256, 0), (3, 256, 2)])
+def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dense_wide_graph, k, threads, blocks):
+    """The streaming wide kernels (16 bytes of vertex words per lane, one row dword per 4 or 2 vertices) on a dense
+    graph, where auto takes them for k = 3 too; each workgroup size, and fewer resident workgroups than slots, is
+    forced through a tuned-plan entry. The trajectory must be the oracle's."""
+    g = dense_wide_graph
+    kw = dict(k=k, population=4, iterations=2, early_stop=2, seed=5, gww=0.25)
+    if threads == 1024 and blocks == 0:
+        got, want = run_both(cga, orc, g, cpu_threads, **kw)
+    else:
+        descent = 3 if k == 2 else 4
+        got, want = run_with_plan(cga, orc, g, cpu_threads, tmp_path,
+                                  f"{g.n} {k} 4 1 {descent} {threads} 0 0 0 {blocks}", **kw)
+    check_same(got, want, f"dense wide k={k} threads={threads} blocks={blocks}")
+    assert (got["descent_name"], got["threads"]) == ("k2-wide" if k == 2 else "k3-wide", threads)
+
+
+@pytest.mark.parametrize("n,p", [(40, 0.3), (64, 0.4), (65, 0.2), (257, 0.4), (1000, 0.05)])
+def test_chunk_kernel_small_graphs(cga, orc, cpu_threads, tmp_path, n, p):
+    """The k = 3 chunk-summary kernel below and at one chunk (n = 40, 64), one vertex into a second chunk (65) and on
+    graphs too dense for auto to pick it, forced through a tuned wide-path plan (ls_kernel = wide)."""
+    g = cga.erdos_renyi(n, p, n)
+    lds = (n + 63) // 64 * 32
+    got, want = run_with_plan(cga, orc, g, cpu_threads, tmp_path, f"{n} 3 6 1 7 1024 0 0 {lds} 0", k=3,
+                              population=6, iterations=4, early_stop=4, seed=7, gww=0.1, ls_kernel="wide")
+    check_same(got, want, f"chunk kernel n={n}")
+    assert got["descent_name"] == "k3-chunk"
+
+
+def test_wide_kernel_choice_follows_density(cga):
+    """auto on the wide path at k = 3: the chunk-summary kernel on a sparse graph, the streaming kernel once twice the
+    mean degree exceeds the number of 64-vertex chunks."""
+    sparse = cga.solve_pbils(cga.chung_lu(40000, 2.5, 16.0, 9), k=3, population=2, iterations=1, early_stop=1)
+    dense = cga.solve_pbils(cga.erdos_renyi(40000, 0.02, 9), k=3, population=2, iterations=1, early_stop=1)
+    assert (sparse["descent_name"], dense["descent_name"]) == ("k3-chunk", "k3-wide")
+
+
 @pytest.mark.parametrize("threads", [256, 1024])
 def test_wide_chunk_kernel_with_several_chunks_per_thread(cga, orc, cpu_threads, tmp_path, threads):
     """The k = 3 chunk-summary kernel when a thread owns several 64-vertex chunks (n = 70000: 1094 chunks): across
