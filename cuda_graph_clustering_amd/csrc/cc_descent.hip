@@ -517,18 +517,21 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   }
   const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;
-  int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start);
+  // Moves counted in 32 bits and compared in scalar registers, the error written after the loop, so that every exit
+  // of the move loop is a scalar branch (n <= 32767 keeps f_start, and with it the cap, below 2^30).
+  const int cap = static_cast<int>(MoveCap(f_start));
+  int applied = 0;
+  bool capped = false;
 
   // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
   auto scan = [&](int parity) {
-    int zmax = kPadY;
+    int zmax = 0;
 #pragma unroll
     for (int j = 0; j < kVpt; ++j) {
       int ymax = y[j][0];
 #pragma unroll
       for (int c = 1; c < K; ++c) ymax = max(ymax, y[j][c]);
-      zmax = max(zmax, ymax + qa[j]);
+      zmax = j == 0 ? ymax + qa[j] : max(zmax, ymax + qa[j]);
     }
     const int x = -zmax;  // ((delta - 1) << Q) | (j << 2b) | (a << b) | c
     cand[parity][tid] = static_cast<Cand>(x & ((1 << kSlotShift) - 1));
@@ -550,7 +553,8 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
   // three rotating words, no reduction after the barrier) gained 0.8 % alone but lost 3 % together with the swaps,
   // so the per-wave partials stay.
   while (true) {
-    const uint32_t wave_best = WaveMinSwap(key);
+    const uint32_t wave_best_v = WaveMinSwapAll(key);
+    const uint32_t wave_best = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave_best_v)));
     // Each wave touches the cache lines of its own best candidate's row (one dword per 128-byte line) before the
     // barrier: the winner is one of these candidates, so its row is on its way while the workgroup minimum is
     // formed, and the runners-up are often the next moves' winners. The value is consumed, without effect, one move
@@ -563,7 +567,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
       const GlobalWords32 cand_row = (GlobalWords32)(row_base + static_cast<size_t>(cand_v) * row_words);
       warm_next = cand_row[min(lane * 32, 2 * row_words - 1)];
     }
-    const uint32_t best = BlockMinOfWaves<T / kWave>(wave_best, partial[parity]);
+    const uint32_t best = BlockMinOfWaves<T / kWave>(wave_best_v, partial[parity]);
     if (best == ~0u) break;
     const int delta = static_cast<int>(best >> 15) - 65535 + 1;
     const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(best & 0x7FFFu));
@@ -651,12 +655,13 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     key = scan(parity);
     objective += delta;
     if (++applied >= cap) {
-      if (tid == 0) {
-        ctl->error = 1;
-        ctl->stopped = 1;
-      }
+      capped = true;
       break;
     }
+  }
+  if (capped && tid == 0) {
+    ctl->error = 1;
+    ctl->stopped = 1;
   }
 
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
@@ -743,8 +748,11 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
   }
   const int64_t f_start = Uniform(r.pop.f[src]);
   int64_t objective = f_start;
-  int64_t applied = 0;
-  const int64_t cap = MoveCap(f_start);
+  // Moves counted in 32 bits and compared in scalar registers, the error written after the loop, so that every exit
+  // of the move loop is a scalar branch (n <= 32767 keeps f_start, and with it the cap, below 2^30).
+  const int cap = static_cast<int>(MoveCap(f_start));
+  int applied = 0;
+  bool capped = false;
 
   // The thread's key: ((delta - 1 + 65535) << 15) | v of its best improving move, or ~0.
   auto scan = [&](int parity) {
@@ -833,12 +841,13 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     key = scan(parity);
     objective += delta;
     if (++applied >= cap) {
-      if (tid == 0) {
-        ctl->error = 1;
-        ctl->stopped = 1;
-      }
+      capped = true;
       break;
     }
+  }
+  if (capped && tid == 0) {
+    ctl->error = 1;
+    ctl->stopped = 1;
   }
 
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;

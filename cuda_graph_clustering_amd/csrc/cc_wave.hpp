@@ -97,16 +97,15 @@ __device__ __forceinline__ T WaveMin(T x) {
   return Min(Min(ReadLane(x, 0), ReadLane(x, 16)), Min(ReadLane(x, 32), ReadLane(x, 48)));
 }
 
-// The same for 32-bit keys with gfx950's lane swaps instead of four v_readlane: after the row minimum,
-// v_permlane32_swap pairs lane i with lane i + 32 and v_permlane16_swap row r with row r ^ 1, so every lane ends
-// with the minimum of the wave, which one v_readfirstlane moves to a scalar register.
-__device__ __forceinline__ uint32_t WaveMinSwap(uint32_t x) {
+// The minimum of a wave's 32-bit keys in every lane, with gfx950's lane swaps instead of four v_readlane: after the
+// row minimum, v_permlane32_swap pairs lane i with lane i + 32 and v_permlane16_swap row r with row r ^ 1, so every
+// lane ends with the minimum of the wave (one v_readfirstlane moves it to a scalar register).
+__device__ __forceinline__ uint32_t WaveMinSwapAll(uint32_t x) {
   x = RowMin(x);
   const auto half = __builtin_amdgcn_permlane32_swap(x, x, false, false);
   x = min(static_cast<uint32_t>(half[0]), static_cast<uint32_t>(half[1]));
   const auto rows = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-  x = min(static_cast<uint32_t>(rows[0]), static_cast<uint32_t>(rows[1]));
-  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+  return min(static_cast<uint32_t>(rows[0]), static_cast<uint32_t>(rows[1]));
 }
 
 // Minimum over a workgroup of kWaves waves (kWaves <= 16) from each wave's own minimum, one barrier. `partial` is
