@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 # of LDS bank conflicts (SQ_LDS_BANK_CONFLICT); fabric: bytes moved beyond L2, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 # (KiB -> bytes; the factors from a 1 GiB device copy, profiles/copy_calibration_1gib.txt).
 MEASURED = {
-    ("c2", "registers", 1024, 10): dict(valu=2192.2, lds=224.0, lds_conflict=0.0, fabric=869.6,
+    ("c2", "registers", 1024, 10): dict(valu=2064.2, lds=224.0, lds_conflict=0.0, fabric=877.3,
                                         profile="profiles/descent_registers_sbm_n10000_k5.txt"),
     ("c5", "k3-chunk", 1024, 0): dict(valu=8539.0, lds=301.2, lds_conflict=29.0, fabric=29824.0,
                                       profile="profiles/descent_k3chunk_chunglu_n100000_k3.txt"),
@@ -530,7 +530,7 @@ def bench_scale(cga, args):
         g = cga.erdos_renyi(n, density, 7)
         note(f"scale: ER n={n} p={density} built in {time.perf_counter() - t0:.1f} s ({g.edges} edges)")
         with cga.DeviceGraph(g) as dg:
-            for k in (2, 3):
+            for k in (int(x) for x in args.scale_ks.split(",")):
                 runs = {}
                 for iters in (1, 3):
                     t0 = time.perf_counter()
@@ -577,6 +577,7 @@ def main():
     ap.add_argument("--tags-objects", type=int, default=60000, help="objects of the --config tags file")
     ap.add_argument("--scale-sizes", default="193734:0.126,330000:0.128",
                     help="--config scale: comma-separated n:density pairs")
+    ap.add_argument("--scale-ks", default="2,3", help="--config scale: comma-separated cluster counts")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the mixed stream; 0 skips it")
     ap.add_argument("--batch-only", action="store_true", help="mixed stream: skip the plain one-by-one leg")
