@@ -1221,6 +1221,9 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   const int lane = tid & 63;
   const int wave = WaveIndex();
   const int chunks = (n + 63) >> 6;
+  constexpr int kRows = kWave / 16;  // dirty chunks a wave recomputes at once
+  const int row_in_wave = lane >> 4;
+  const int quad = lane & 15;
   __shared__ uint64_t partial[2][kWaves];
   __shared__ int32_t dirty_count[2];
   __shared__ int32_t src_sh;
@@ -1237,7 +1240,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
-  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * n);
+  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * WideStride(n, 8));
   // the three cluster sizes (< 2^21) as 21-bit fields of one uniform word: a move updates them with scalar adds
   uint64_t szp = static_cast<uint64_t>(sizes_in[0]) | static_cast<uint64_t>(sizes_in[1]) << 21 |
                  static_cast<uint64_t>(sizes_in[2]) << 42;
@@ -1342,27 +1345,63 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
     }
     __syncthreads();
     const int count = dirty_count[parity];
-    for (int i = wave; i < count; i += kWaves) {
-      const int chunk = list[i];
-      const uint64_t word = row[chunk];  // wave-uniform
-      const int u = chunk * kWave + lane;
-      uint64_t w = 0;
-      bool neighbour = false;
-      int a = 3;
-      if (u < n) {
-        w = st[u];
-        a = static_cast<int>(w >> kK3LabelShift);
-        neighbour = (word >> lane) & 1ull;
-        if (neighbour) w += inc;
-        if (u == v) w = (w & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
-        if (neighbour || u == v) st[u] = w;
+    // Each row of 16 lanes takes one dirty chunk, so a wave recomputes four chunks at once: lane q of the row holds
+    // the chunk's vertices 4 q .. 4 q + 3 (two 16-byte loads), and a pair's chunk maximum is one DPP row maximum
+    // shared by the four chunks. The pairs recomputed are those of the labels changed in any of the four chunks (a
+    // pair recomputed without need gets its unchanged value).
+    for (int base = wave * kRows; base < count; base += kWaves * kRows) {  // wave-uniform
+      const int i = base + row_in_wave;
+      const bool active = i < count;
+      const int chunk = active ? list[i] : 0;
+      const uint32_t bits = active ? static_cast<uint32_t>(row[chunk] >> (4 * quad)) & 0xFu : 0u;
+      const int u0 = chunk * kWave + 4 * quad;
+      const GlobalPtr<Quad> sq = (GlobalPtr<Quad>)(st + u0);
+      uint64_t w[4] = {0, 0, 0, 0};
+      if (active) {
+        const Quad lo = sq[0], hi = sq[1];
+        __builtin_memcpy(&w[0], &lo, 16);
+        __builtin_memcpy(&w[2], &hi, 16);
       }
-      // Only the pairs of a changed lane's label can change: a neighbour keeps its label (its G moved), the mover
-      // leaves `from` and joins `to`.
-      int labels = chunk == (v >> 6) ? (1 << from) | (1 << to) : 0;
+      int changed = 0;  // labels whose pairs change: a neighbour keeps its label, the mover leaves `from` for `to`
 #pragma unroll
-      for (int c = 0; c < 3; ++c) labels |= __ballot(neighbour && a == c) ? 1 << c : 0;
-      summarize(chunk, w, u < n, labels);
+      for (int e = 0; e < 4; ++e) {
+        const bool valid = active && u0 + e < n;
+        if (valid && ((bits >> e) & 1u)) {
+          w[e] += inc;
+          changed |= 1 << static_cast<int>(w[e] >> kK3LabelShift);
+        }
+        if (valid && u0 + e == v) {
+          w[e] = (w[e] & ~(3ull << kK3LabelShift)) | (static_cast<uint64_t>(to) << kK3LabelShift);
+          changed |= (1 << from) | (1 << to);
+        }
+      }
+      if (changed) {
+        Quad lo, hi;
+        __builtin_memcpy(&lo, &w[0], 16);
+        __builtin_memcpy(&hi, &w[2], 16);
+        sq[0] = lo;
+        sq[1] = hi;
+      }
+      int labels = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) labels |= __ballot((changed >> c) & 1) ? 1 << c : 0;
+#pragma unroll
+      for (int p = 0; p < kPairs3; ++p) {
+        if (!((labels >> (p >> 1)) & 1)) continue;  // uniform
+        const int src = p >> 1, tgt = PairTarget3(p);
+        uint32_t best = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int a = static_cast<int>(w[e] >> kK3LabelShift);
+          const int ga = static_cast<int>((w[e] >> (kK3Bits * src)) & kK3Mask);
+          const int gc = static_cast<int>((w[e] >> (kK3Bits * tgt)) & kK3Mask);
+          const uint32_t key = (static_cast<uint32_t>(gc - ga + (1 << kK3Bits)) << 6) |
+                               static_cast<uint32_t>(63 - 4 * quad - e);
+          best = (active && u0 + e < n && a == src) ? max(best, key) : best;
+        }
+        best = RowMax(best);
+        if (active && quad == 0) table[chunk * kPairs3 + p] = best;
+      }
     }
     // the other list's counter was last read before this move's first barrier
     if (tid == 0) dirty_count[parity ^ 1] = 0;

@@ -75,6 +75,22 @@ __device__ __forceinline__ T RowMin(T x) {
   return x;
 }
 
+template <int kCtrl>
+__device__ __forceinline__ uint32_t DppMaxStep(uint32_t x) {
+  const uint32_t other =
+      static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(x), kCtrl, 0xF, 0xF, true));
+  return max(other, x);
+}
+
+// Maximum over each row of 16 lanes, present in every lane of the row.
+__device__ __forceinline__ uint32_t RowMax(uint32_t x) {
+  x = DppMaxStep<kDppXor1>(x);
+  x = DppMaxStep<kDppXor2>(x);
+  x = DppMaxStep<kDppHalfMirror>(x);
+  x = DppMaxStep<kDppMirror>(x);
+  return x;
+}
+
 __device__ __forceinline__ uint64_t ReadLane(uint64_t x, int lane) {
   const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), lane));
   const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), lane));
