@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 MEASURED = {
     ("c2", "registers", 1024, 10): dict(valu=2064.2, lds=224.0, lds_conflict=0.0, fabric=877.3,
                                         profile="profiles/descent_registers_sbm_n10000_k5.txt"),
-    ("c5", "k3-chunk", 1024, 0): dict(valu=7463.9, lds=202.9, lds_conflict=32.1, fabric=29695.0,
+    ("c5", "k3-chunk", 1024, 0): dict(valu=7466.4, lds=202.9, lds_conflict=32.2, fabric=29893.0,
                                       profile="profiles/descent_k3chunk_chunglu_n100000_k3.txt"),
 }
 

@@ -87,6 +87,20 @@ void GeneratorSbm(unsigned n, int blocks, double p_in, double p_out, uint64_t se
   });
 }
 
+// Chung-Lu in O(n + m) by geometric skips (J. C. Miller, A. Hagberg, "Efficient generation of networks with given
+// expected degrees", WAW 2011). The weights fall with the index, so along row u the pair probability
+// p(u, v) = min(1, w_u w_v / S) falls with v: from v with current bound p, floor(log r / log(1 - p)) pairs
+// (r uniform in (0, 1]) are passed over at once, the pair reached is kept with probability p(u, v) / p, and the bound
+// becomes p(u, v) — each pair an edge with probability p(u, v), independently. Row u draws from its own stream
+// StreamSeed(seed, kChungLuStream, u + 1) (53-bit uniforms), so rows are independent work items; each writes the
+// upper triangle of its row, and the edges it found are mirrored afterwards.
+inline double UniformOpenZero(uint64_t& state) {  // (0, 1]
+  return static_cast<double>((SplitMix64(state) >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+inline double UniformOpenOne(uint64_t& state) {  // [0, 1)
+  return static_cast<double>(SplitMix64(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
 void GeneratorChungLu(unsigned n, double gamma, double mean_degree, uint64_t seed, Word* bits, int threads) {
   std::vector<double> w(n);
   const double exponent = -1.0 / (gamma - 1.0);
@@ -99,10 +113,38 @@ void GeneratorChungLu(unsigned n, double gamma, double mean_degree, uint64_t see
   const double scale = n ? total / sum : 0.0;
   for (unsigned i = 0; i < n; ++i) w[i] *= scale;
   const double inv_total = n ? 1.0 / total : 0.0;
-  FillPairs(n, WordsFor(n), StreamSeed(seed, kChungLuStream, 0), bits, threads, [&](unsigned i, unsigned j) {
-    const double q = w[i] * w[j] * inv_total;
+  const uint32_t words = WordsFor(n);
+  auto probability = [&](unsigned u, unsigned v) {
+    const double q = w[u] * w[v] * inv_total;
     return q > 1.0 ? 1.0 : q;
+  };
+  std::vector<std::vector<unsigned>> above(n);  // row u: its neighbours v > u, for the mirror pass
+  ParallelFor(static_cast<int>(n), threads, [&](int item) {
+    const unsigned u = static_cast<unsigned>(item);
+    Word* row = bits + static_cast<size_t>(u) * words;
+    std::fill(row, row + words, Word(0));
+    uint64_t state = StreamSeed(seed, kChungLuStream, static_cast<uint64_t>(u) + 1);
+    unsigned v = u + 1;
+    if (v >= n) return;
+    double p = probability(u, v);
+    while (v < n && p > 0.0) {
+      if (p < 1.0) {
+        const double skip = std::floor(std::log(UniformOpenZero(state)) / std::log1p(-p));
+        if (skip >= static_cast<double>(n - v)) break;
+        v += static_cast<unsigned>(skip);
+      }
+      const double q = probability(u, v);
+      if (UniformOpenOne(state) < q / p) {
+        row[v / kWordBits] |= Word(1) << (v % kWordBits);
+        above[u].push_back(v);
+      }
+      p = q;
+      ++v;
+    }
   });
+  for (unsigned u = 0; u < n; ++u) {
+    for (const unsigned v : above[u]) bits[static_cast<size_t>(v) * words + u / kWordBits] |= Word(1) << (u % kWordBits);
+  }
 }
 
 }  // namespace

@@ -35,8 +35,9 @@ class Graph {
   // Planted partition: vertex v in block v * blocks / n; p_in inside a block, p_out across; ER's draw order on
   // the stream kSbmStream.
   static Graph StochasticBlock(unsigned n, int blocks, double p_in, double p_out, uint64_t seed, int threads = 0);
-  // Chung-Lu: w_i = (i + 1)^(-1 / (gamma - 1)) scaled to mean mean_degree, p_ij = min(1, w_i w_j / (mean_degree n));
-  // ER's draw order on the stream kChungLuStream.
+  // Chung-Lu: w_i = (i + 1)^(-1 / (gamma - 1)) scaled to mean mean_degree, p_ij = min(1, w_i w_j / (mean_degree n)),
+  // each pair independently; drawn in O(n + m) by geometric skips along each row, row u on the stream
+  // StreamSeed(seed, kChungLuStream, u + 1).
   static Graph ChungLu(unsigned n, double gamma, double mean_degree, uint64_t seed, int threads = 0);
   // Adopts a bitset (n rows of WordsPerRow words) and counts its edges.
   static Graph FromBits(std::vector<Word> bits, unsigned n);

@@ -203,15 +203,14 @@ static inline float pair_draw(uint64_t state0, uint32_t i, uint32_t j) {
   return (float)(u >> 8) * (1.0f / 16777216.0f);
 }
 
-typedef enum { GEN_ER, GEN_SBM, GEN_CL } gen_kind;
+typedef enum { GEN_ER, GEN_SBM } gen_kind;
 
 typedef struct {
   gen_kind kind;
   uint32_t n, words;
   uint64_t state0;
-  double p, p_in, p_out, inv_total;
+  double p, p_in, p_out;
   int blocks;
-  const double* weights;
   uint64_t* bits;
 } gen_ctx;
 
@@ -222,10 +221,6 @@ static inline double gen_probability(const gen_ctx* g, uint32_t i, uint32_t j) {
       const uint32_t bi = (uint32_t)(((uint64_t)i * (uint64_t)g->blocks) / g->n);
       const uint32_t bj = (uint32_t)(((uint64_t)j * (uint64_t)g->blocks) / g->n);
       return bi == bj ? g->p_in : g->p_out;
-    }
-    case GEN_CL: {
-      double q = g->weights[i] * g->weights[j] * g->inv_total;
-      return q > 1.0 ? 1.0 : q;
     }
   }
   return 0.0;
@@ -285,10 +280,61 @@ uint64_t orc_graph_sbm(uint32_t n, int blocks, double p_in, double p_out, uint64
   return gen_run(&g, threads);
 }
 
+/* Chung-Lu in O(n + m), by the edge-skipping method of J. C. Miller and A. Hagberg, "Efficient generation of
+ * networks with given expected degrees" (WAW 2011). The weights fall with the index, so along row u the pair
+ * probability p(u, v) = min(1, w_u w_v / S) falls with v. From the current v with probability p the number of pairs
+ * skipped is geometric, floor(log(r) / log(1 - p)) for r uniform in (0, 1]; the candidate reached is kept with
+ * probability p(u, v) / p (a uniform in [0, 1) below it), and p becomes p(u, v). Every pair (u, v) is then an edge with
+ * probability p(u, v), independently of the others. Row u draws 53-bit uniforms from the stream
+ * StreamSeed(seed, ORC_CHUNG_LU_STREAM, u + 1), so rows are independent work items: each worker draws the pairs
+ * above the diagonal of its rows, and the lower triangle is mirrored afterwards. */
+typedef struct {
+  uint32_t n, words;
+  uint64_t seed;
+  const double* weights;
+  double inv_total;
+  uint64_t* bits;
+} cl_ctx;
+
+static inline double cl_uniform_open0(uint64_t* state) { /* (0, 1] */
+  return (double)((orc_splitmix64(state) >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+static inline double cl_uniform_open1(uint64_t* state) { /* [0, 1) */
+  return (double)(orc_splitmix64(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static inline double cl_probability(const cl_ctx* c, uint32_t u, uint32_t v) {
+  const double q = c->weights[u] * c->weights[v] * c->inv_total;
+  return q > 1.0 ? 1.0 : q;
+}
+
+static void cl_row(void* ctx, int item, int worker) {
+  (void)worker;
+  const cl_ctx* c = (const cl_ctx*)ctx;
+  const uint32_t u = (uint32_t)item;
+  const uint32_t n = c->n;
+  uint64_t* row = c->bits + (size_t)u * c->words;
+  memset(row, 0, sizeof(uint64_t) * c->words);
+  uint64_t state = orc_stream_seed(c->seed, ORC_CHUNG_LU_STREAM, (uint64_t)u + 1);
+  uint32_t v = u + 1;
+  if (v >= n) return;
+  double p = cl_probability(c, u, v);
+  while (v < n && p > 0.0) {
+    if (p < 1.0) {
+      const double skip = floor(log(cl_uniform_open0(&state)) / log1p(-p));
+      if (skip >= (double)(n - v)) break;
+      v += (uint32_t)skip;
+    }
+    const double q = cl_probability(c, u, v);
+    if (cl_uniform_open1(&state) < q / p) row[v >> 6] |= 1ull << (v & 63);
+    p = q;
+    ++v;
+  }
+}
+
 uint64_t orc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_t seed, uint64_t* bits,
                             int threads) {
-  gen_ctx g;
-  memset(&g, 0, sizeof(g));
   double* w = (double*)malloc(sizeof(double) * (n ? n : 1));
   const double exponent = -1.0 / (gamma - 1.0);
   double sum = 0.0;
@@ -299,15 +345,28 @@ uint64_t orc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64
   const double total = mean_degree * (double)n;
   const double scale = n ? total / sum : 0.0;
   for (uint32_t i = 0; i < n; ++i) w[i] *= scale;
-  g.kind = GEN_CL;
-  g.n = n;
-  g.weights = w;
-  g.inv_total = n ? 1.0 / total : 0.0;
-  g.state0 = orc_stream_seed(seed, ORC_CHUNG_LU_STREAM, 0);
-  g.bits = bits;
-  const uint64_t edges = gen_run(&g, threads);
+  cl_ctx c;
+  c.n = n;
+  c.words = orc_words_per_row(n);
+  c.seed = seed;
+  c.weights = w;
+  c.inv_total = n ? 1.0 / total : 0.0;
+  c.bits = bits;
+  orc_parallel_for((int)n, orc_resolve_threads(threads), cl_row, &c);
+  /* mirror: every bit (u, v) with v > u sets (v, u) */
+  for (uint32_t u = 0; u < n; ++u) {
+    const uint64_t* row = bits + (size_t)u * c.words;
+    for (uint32_t wd = (u + 1) >> 6; wd < c.words; ++wd) {
+      uint64_t x = row[wd];
+      while (x) {
+        const uint32_t v = (wd << 6) + (uint32_t)__builtin_ctzll(x);
+        x &= x - 1;
+        if (v > u) bits[(size_t)v * c.words + (u >> 6)] |= 1ull << (u & 63);
+      }
+    }
+  }
   free(w);
-  return edges;
+  return orc_edge_count(bits, n, c.words);
 }
 
 /* ------------------------------------------------------------------------------------------------------------ */

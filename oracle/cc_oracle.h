@@ -17,7 +17,8 @@
  *   search     — incremental == rebuild after random moves; local-optimum property; known optima.
  *   GWW        — RankSlots/WinnerCopies known answers (src/cc_verify.cpp:379-383).
  *   generators — ER identical to the reference generator; SBM and Chung-Lu are defined here (they do not
- *                exist in the reference) and pinned by edge-count statistics and the closed-form draw rule.
+ *                exist in the reference) and pinned by edge-count statistics, the closed-form draw rule (SBM) and
+ *                per-pair edge frequencies over many seeds against p_ij (Chung-Lu).
  */
 #ifndef CC_ORACLE_H
 #define CC_ORACLE_H
@@ -50,7 +51,9 @@ uint64_t orc_graph_erdos_renyi(uint32_t n, double p, uint64_t seed, uint64_t* bi
 /* Planted partition: block(v) = v*blocks/n; same lower-triangle draw order as ER, own stream. */
 uint64_t orc_graph_sbm(uint32_t n, int blocks, double p_in, double p_out, uint64_t seed, uint64_t* bits,
                        int threads);
-/* Chung-Lu: w_i = (i+1)^(-1/(gamma-1)) scaled to mean degree d; p_ij = min(1, w_i w_j / (d n)). */
+/* Chung-Lu: w_i = (i+1)^(-1/(gamma-1)) scaled to mean degree d; p_ij = min(1, w_i w_j / (d n)), each pair
+ * independently; drawn in O(n + m) by geometric skips along each row (Miller and Hagberg 2011), row u on the stream
+ * StreamSeed(seed, 0xC1A9E5, u + 1). */
 uint64_t orc_graph_chung_lu(uint32_t n, double gamma, double mean_degree, uint64_t seed, uint64_t* bits,
                             int threads);
 uint64_t orc_edge_count(const uint64_t* bits, uint32_t n, uint32_t words);

@@ -248,6 +248,44 @@ def test_chung_lu_mean_degree(orc):
     assert deg[:10].mean() > 10 * deg[-1000:].mean()  # heavy head, light tail
 
 
+@pytest.mark.parametrize("n,mean_degree", [(9, 2.0), (7, 4.5)])
+def test_chung_lu_pairs_follow_their_probabilities(orc, n, mean_degree):
+    # The O(n + m) sampler (geometric skips along each row) must make every pair an edge with its own probability
+    # p_ij = min(1, w_i w_j / (d n)), independently. Over 12000 seeds each pair's frequency lies within 5 standard
+    # errors of p_ij (pairs with p_ij = 1 always joined), and two pairs of one row, which share the row's skips, are
+    # joined together as often as p_ij p_ik says. At mean degree 4.5 on 7 vertices the heaviest pairs are capped at 1.
+    trials = 12000
+    w = np.arange(1, n + 1, dtype=np.float64) ** (-1.0 / 1.5)
+    w *= mean_degree * n / w.sum()
+    p = np.minimum(1.0, np.outer(w, w) / (mean_degree * n))
+    count = np.zeros((n, n))
+    both = 0
+    for seed in range(trials):
+        adj = orc.chung_lu(n, 2.5, mean_degree, seed, threads=1).dense()
+        count += adj
+        both += int(adj[1, 2] and adj[1, 4])
+    freq = count / trials
+    for i in range(n):
+        for j in range(i + 1, n):
+            se = np.sqrt(p[i, j] * (1 - p[i, j]) / trials)
+            assert abs(freq[i, j] - p[i, j]) <= 5 * se + 1e-12, (i, j, freq[i, j], p[i, j])
+    pb = p[1, 2] * p[1, 4]
+    assert abs(both / trials - pb) <= 5 * np.sqrt(pb * (1 - pb) / trials) + 1e-12
+    if mean_degree > 4:
+        assert p[0, 1] == 1.0 and freq[0, 1] == 1.0
+
+
+def test_chung_lu_large_graph_is_cheap_and_has_its_mean_degree(orc):
+    # O(n + m): the wide-path workload's graph (n = 100000, mean degree 64) in well under a second of one core's work
+    # would be 5e9 pair draws in the pairwise order; its mean degree is the target less what capping removes.
+    import time
+
+    t0 = time.perf_counter()
+    g = orc.chung_lu(100000, 2.5, 64.0, 4, threads=1)
+    assert time.perf_counter() - t0 < 20.0
+    assert 0.95 * 64 < 2 * g.edges / g.n < 64.0
+
+
 def test_er_draw_is_the_reference_sequence(orc):
     # ER is defined by the reference's sequential draw order (src/cc_graph.cpp:168-177); the oracle evaluates
     # pair (i, j) as draw number i(i-1)/2 + j of the stream. Check that against a sequential walk of the stream.
