@@ -16,6 +16,7 @@
 // move i the buffer i & 1 holds the sizes before move i - 1 and the correction for move i - 1 comes from registers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -871,23 +872,29 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
 
 template <typename Word>
 using GlobalPtr = __attribute__((address_space(1))) Word*;
-// 16 bytes of vertex words, one global_load_dwordx4 / global_store_dwordx4 per lane.
+template <typename Word>
+using LdsPtr = __attribute__((address_space(3))) Word*;
+// 16 bytes of vertex words, one global_load_dwordx4 / global_store_dwordx4 (ds_read_b128 / ds_write_b128) per lane.
 using Quad = uint32_t __attribute__((ext_vector_type(4)));
 
 // One pass of a streaming wide-path descent over a slot's packed vertex words (Word: uint32_t at k = 2, uint64_t at
 // k = 3) after a move, 16 bytes per lane. With kV = 16 / sizeof(Word), a wave's tile is kStreamLoads loads: lane l of
 // load j holds the kV consecutive vertices u = tile * kTile + (64 j + l) kV + i, whose row bits are bits
 // (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tile t belongs to wave t % kWaves (the
-// same owner as in ForEachOwned), and a wave loads its next tile before it consumes the current one, so 2 x 64 B of
-// words and two row dwords per lane are in flight behind the arithmetic. Every word of a neighbour of the moved
-// vertex gets `inc`, the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of itself), a
-// 16-byte group with a changed word is written back, and visit(w, u) sees every vertex u < n in increasing order
-// within the thread. `st` is the slot's base (16-byte aligned, padded to whole tiles: the padding
-// words take increments and are written back, never visited). Row dwords of the last tile may lie beyond the row
-// (the next row, or the zero tail after the last row); their bits reach only padding words.
+// same owner as in ForEachOwned). Tiles t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile,
+// so they hold no vertex >= n, and resident <= kWaves * kResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its
+// resident tiles and the loads of its first streamed tile, then updates its resident tiles while those loads are in
+// flight, then streams the rest, loading its next tile before it consumes the current one (2 x 64 B of words and
+// two row dwords per lane in flight behind the arithmetic). Every word of a neighbour of the moved vertex gets `inc`,
+// the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of itself), a 16-byte group with
+// a changed word is written back, and visit(w, u) sees every vertex u < n in increasing order within the thread.
+// `st` is the slot's base (16-byte aligned, padded to whole tiles: the padding words take increments and are written
+// back, never visited). Row dwords of the last tile may lie beyond the row (the next row, or the zero tail after the
+// last row); their bits reach only padding words.
 template <int T, typename Word, typename Relabel, typename Visit>
-__device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, GlobalWords32 row, int n, int wave, int lane, Word inc,
-                                           int mover, Relabel&& relabel, Visit&& visit) {
+__device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res, int resident, GlobalWords32 row,
+                                           int n, int wave, int lane, Word inc, int mover, Relabel&& relabel,
+                                           Visit&& visit) {
   constexpr int kV = 16 / static_cast<int>(sizeof(Word));
   constexpr int kJ = kStreamLoads;
   constexpr int kTile = StreamTile(sizeof(Word));
@@ -900,21 +907,24 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, GlobalWords32 row
   const int mover_load = (mover - mover_tile * kTile) / (kWave * kV);
   const int mover_lane = (mover / kV) % kWave;
   const int mover_word = mover % kV;
-  Quad cw[kJ], nw[kJ];
-  uint32_t cr[kJ], nr[kJ];
+  auto quad = [&](int t, int j) { return static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane; };
+  auto load_rows = [&](int t, uint32_t* rr) {
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) rr[j] = row[((t * kTile + j * kWave * kV) >> 5) + lane_dword];
+  };
   auto load = [&](int t, Quad* w, uint32_t* rr) {
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
-      w[j] = st[static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane];
+      w[j] = st[quad(t, j)];
       rr[j] = row[((t * kTile + j * kWave * kV) >> 5) + lane_dword];
     }
   };
-  auto consume = [&](int t, auto full) {
+  auto consume = [&](int t, const Quad* w, const uint32_t* rr, auto full, auto in_lds) {
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
-      const uint32_t bits = cr[j] >> lane_bit;
+      const uint32_t bits = rr[j] >> lane_bit;
       Word ws[kV];
-      __builtin_memcpy(ws, &cw[j], 16);
+      __builtin_memcpy(ws, &w[j], 16);
 #pragma unroll
       for (int i = 0; i < kV; ++i) ws[i] += inc & (Word(0) - static_cast<Word>((bits >> i) & 1u));
       bool changed = (bits & ((1u << kV) - 1u)) != 0;
@@ -928,7 +938,11 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, GlobalWords32 row
       if (changed) {
         Quad out;
         __builtin_memcpy(&out, ws, 16);
-        st[static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane] = out;
+        if constexpr (decltype(in_lds)::value) {
+          res[quad(t, j)] = out;
+        } else {
+          st[quad(t, j)] = out;
+        }
       }
       const int u0 = t * kTile + (j * kWave + lane) * kV;
 #pragma unroll
@@ -937,14 +951,32 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, GlobalWords32 row
       }
     }
   };
-  int t = wave;
+  uint32_t rres[kResidentPerWave][kJ];
+#pragma unroll
+  for (int i = 0; i < kResidentPerWave; ++i) {
+    if (wave + i * kWaves < resident) load_rows(wave + i * kWaves, rres[i]);
+  }
+  // this wave's first streamed tile: the smallest t >= resident with t % kWaves == wave
+  int t = wave + kWaves * ((max(resident - wave, 0) + kWaves - 1) / kWaves);
+  Quad cw[kJ], nw[kJ];
+  uint32_t cr[kJ], nr[kJ];
   if (t < tiles) load(t, cw, cr);
+#pragma unroll
+  for (int i = 0; i < kResidentPerWave; ++i) {
+    const int tr = wave + i * kWaves;
+    if (tr < resident) {
+      Quad w[kJ];
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) w[j] = res[quad(tr, j)];
+      consume(tr, w, rres[i], std::true_type{}, std::true_type{});
+    }
+  }
   for (; t < tiles; t += kWaves) {
     if (t + kWaves < tiles) load(t + kWaves, nw, nr);
     if ((t + 1) * kTile <= n) {  // wave-uniform
-      consume(t, std::true_type{});
+      consume(t, cw, cr, std::true_type{}, std::false_type{});
     } else {
-      consume(t, std::false_type{});
+      consume(t, cw, cr, std::false_type{}, std::false_type{});
     }
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
@@ -985,7 +1017,7 @@ __device__ __forceinline__ int StreamOwner(int v) {
 // neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored from w and the degrees at the
 // end.
 template <int T>
-__device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
+__device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int resident_tiles) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1002,6 +1034,11 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 2 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   uint32_t* state = reinterpret_cast<uint32_t*>(r.work) + static_cast<size_t>(s) * WideStride(n, 4);
+  // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
+  const int resident = min(resident_tiles, min(n / StreamTile(4), kWaves * kResidentPerWave));
+  const int res_words = resident * StreamTile(4);
+  extern __shared__ __align__(16) uint8_t dyn[];
+  const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
   int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
   const int64_t f_start = r.pop.f[src];
 
@@ -1018,7 +1055,11 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   };
   ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
     const uint32_t w = static_cast<uint32_t>(2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1));
-    state[u] = w;
+    if (u < res_words) {
+      res[u] = w;
+    } else {
+      state[u] = w;
+    }
     consider(w, u);
   });
   uint64_t key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
@@ -1045,7 +1086,8 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_u = -1;
-    StreamMove<T, uint32_t>(st, row, n, wave, lane, inc, v, [](uint32_t w) { return w ^ 1u; }, consider);
+    StreamMove<T, uint32_t>(st, (LdsPtr<Quad>)res, resident, row, n, wave, lane, inc, v,
+                            [](uint32_t w) { return w ^ 1u; }, consider);
     key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
     objective += delta;
     if (++applied >= cap) {
@@ -1060,7 +1102,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s) {
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
   ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
-    const int w = static_cast<int>(state[u]);
+    const int w = static_cast<int>(u < res_words ? res[u] : state[u]);
     const int e = w >> 1;
     const int degree = r.deg[u];
     lab_out[u] = static_cast<uint8_t>(w & 1);
@@ -1085,7 +1127,7 @@ constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
 constexpr int kK3LabelShift = 62;
 
 template <int T>
-__device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
+__device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int resident_tiles) {
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1103,6 +1145,11 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
   uint64_t* state = r.work + static_cast<size_t>(s) * WideStride(n, 8);
+  // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
+  const int resident = min(resident_tiles, min(n / StreamTile(8), kWaves * kResidentPerWave));
+  const int res_words = resident * StreamTile(8);
+  extern __shared__ __align__(16) uint8_t dyn[];
+  const LdsPtr<uint64_t> res = (LdsPtr<uint64_t>)dyn;
   int s0 = sizes_in[0], s1 = sizes_in[1], s2 = sizes_in[2];
   const int64_t f_start = r.pop.f[src];
 
@@ -1130,7 +1177,11 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
     const uint64_t w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
                        static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
                        static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
-    state[u] = w;
+    if (u < res_words) {
+      res[u] = w;
+    } else {
+      state[u] = w;
+    }
     consider(w, u);
   });
   uint64_t key = key_of();
@@ -1161,7 +1212,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
     best_d = 0;
     best_move = -1;
     StreamMove<T, uint64_t>(
-        st, row, n, wave, lane, inc, v,
+        st, (LdsPtr<Quad>)res, resident, row, n, wave, lane, inc, v,
         [label_to](uint64_t w) { return (w & ~(3ull << kK3LabelShift)) | label_to; }, consider);
     key = key_of();
     objective += delta;
@@ -1177,7 +1228,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s) {
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
   ForEachOwned<T, 8>(n, wave, lane, [&](int u) {
-    const uint64_t w = state[u];
+    const uint64_t w = u < res_words ? res[u] : state[u];
     g_out[u] = static_cast<int32_t>(w & kK3Mask);
     g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
     g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
@@ -1491,13 +1542,13 @@ __global__ void __launch_bounds__(T) KernelDescentK3Packed(const RunDev* runs, I
 }
 
 template <int T>
-__global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, ItemList list) {
-  Persistent(runs, list, [](const RunDev& r, int s) { DescentK2WideBody<T>(r, s); });
+__global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, ItemList list, int resident_tiles) {
+  Persistent(runs, list, [resident_tiles](const RunDev& r, int s) { DescentK2WideBody<T>(r, s, resident_tiles); });
 }
 
 template <int T>
-__global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list) {
-  Persistent(runs, list, [](const RunDev& r, int s) { DescentK3WideBody<T>(r, s); });
+__global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list, int resident_tiles) {
+  Persistent(runs, list, [resident_tiles](const RunDev& r, int s) { DescentK3WideBody<T>(r, s, resident_tiles); });
 }
 
 template <int T>
@@ -1740,10 +1791,21 @@ bool GoRegs(const Go& go) {
   }
 }
 
+// The streaming wide-path kernels take the number of resident tiles their dynamic LDS holds as a launch argument, so
+// the tiles a descent keeps in LDS never exceed what the launch allocated (a batch launch allocates the largest LDS
+// of its runs' plans; a tuned plan may hold none).
 template <template <int> class Wide>
 bool GoWide(const Go& go) {
-  if (go.plan->threads == 256) return go.Run(Wide<256>::kernel, 256, 0);
-  if (go.plan->threads == 1024) return go.Run(Wide<1024>::kernel, 1024, 0);
+  const size_t lds = go.plan->lds_bytes;
+  const int resident = static_cast<int>(lds / kStreamTileBytes);
+  auto run = [&](auto kernel, int threads) {
+    if (go.mode == GoMode::kProbe) return true;
+    AllowLds(kernel, lds);
+    if (go.mode == GoMode::kLaunch) kernel<<<go.Blocks(), threads, lds, go.stream>>>(go.runs, *go.list, resident);
+    return true;
+  };
+  if (go.plan->threads == 256) return run(Wide<256>::kernel, 256);
+  if (go.plan->threads == 1024) return run(Wide<1024>::kernel, 1024);
   return false;
 }
 
@@ -1819,6 +1881,14 @@ void LaunchDescent(const RunDev* runs, const ItemList& items, const DescentPlan&
 
 size_t GenericLdsBytes(int n, int k, int wide) {
   return Align16((wide ? 4 : 2) * static_cast<size_t>(k) * n) + Align16(static_cast<size_t>(n));
+}
+
+size_t WideResidentLdsBytes(int n, int k, int threads) {
+  const int word_bytes = k == 2 ? 4 : 8;
+  const int64_t tiles = n / StreamTile(word_bytes);  // whole tiles only
+  const int64_t fit = static_cast<int64_t>(DescentLdsLimit() / kStreamTileBytes);
+  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * kResidentPerWave;
+  return static_cast<size_t>(std::min({tiles, fit, per_waves})) * kStreamTileBytes;
 }
 
 size_t K3ChunkLdsBytes(int n) {

@@ -101,6 +101,10 @@ __host__ __device__ constexpr int StreamTile(int word_bytes) { return 64 * (16 /
 __host__ __device__ constexpr int64_t WideStride(int n, int word_bytes) {
   return (static_cast<int64_t>(n) + StreamTile(word_bytes) - 1) / StreamTile(word_bytes) * StreamTile(word_bytes);
 }
+// A tile is 4 KiB at either word size. The first tiles of a slot may live in the workgroup's LDS instead of HBM (at
+// most kResidentPerWave per wave, as many as the LDS holds): they are never streamed.
+constexpr int kStreamTileBytes = 64 * 16 * kStreamLoads;
+constexpr int kResidentPerWave = 2;
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 
@@ -179,6 +183,10 @@ bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 size_t K3ChunkLdsBytes(int n);
+// LDS of the streaming wide-path descents (k = 2, k = 3) with `threads` per workgroup: the slot's first tiles, as
+// many as fit (at most kResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
+// always streamed).
+size_t WideResidentLdsBytes(int n, int k, int threads);
 int RegistersVpt(int n, int k, int* threads);
 // Vertices per thread of the packed k = 3 descent (1024 threads) for n <= 32767: the smallest of 16, 20, ..., 32
 // that covers n.

@@ -270,6 +270,7 @@ DescentPlan PlanDescent(uint32_t n_, int k, const cc_pbils_params& p, uint64_t e
   if (wide && p.ls_kernel != CC_LS_GLOBAL && (k == 2 || k == 3)) {
     plan.descent = k == 2 ? Descent::kK2Wide : Descent::kK3Wide;
     plan.threads = n >= 8192 ? 1024 : 256;
+    plan.lds_bytes = WideResidentLdsBytes(n, k, plan.threads);
     if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit() && SparseForChunks(n_, edges)) {
       plan.descent = Descent::kK3Chunk;
       plan.lds_bytes = K3ChunkLdsBytes(n);

@@ -92,17 +92,19 @@ std::vector<DescentPlan> Candidates(uint32_t n_, int k, int population, bool wid
     if (p.vpt > 0 && DescentSupported(p)) out.push_back(p);
   }
   if (wide && (k == 2 || k == 3)) {
-    std::vector<int> resident = {0};
+    std::vector<int> block_counts = {0};  // 0: one workgroup per item
     for (int b : {population / 2, population / 4}) {
-      if (b >= 8 && b < std::min(population, cus)) resident.push_back(b);
+      if (b >= 8 && b < std::min(population, cus)) block_counts.push_back(b);
     }
     for (int t : {256, 1024}) {
-      for (int b : resident) {
+      for (int b : block_counts) {
         DescentPlan p = base;
         p.descent = k == 2 ? Descent::kK2Wide : Descent::kK3Wide;
         p.threads = t;
         p.blocks = b;
-        AddIfSupported(out, p);
+        AddIfSupported(out, p);  // every tile streamed
+        p.lds_bytes = WideResidentLdsBytes(n, k, t);
+        AddIfSupported(out, p);  // the first tiles in LDS
       }
       if (k == 3 && K3ChunkLdsBytes(n) <= DescentLdsLimit()) {
         DescentPlan p = base;

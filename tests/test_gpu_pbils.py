@@ -299,21 +299,30 @@ def dense_wide_graph(cga):
     return cga.erdos_renyi(33001, 0.12, 21)
 
 
-@pytest.mark.parametrize("k,threads,blocks", [(2, 1024, 0), (3, 1024, 0), (2, 256, 0), (3, 256, 2)])
-def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dense_wide_graph, k, threads, blocks):
+# (k, threads, blocks, tiles in LDS): None = the default plan's (every whole tile that fits, at most two per wave);
+# k = 2 at 1024 threads keeps 32 of the 33 tiles of n = 33001 (the partial last one is streamed), k = 3 keeps 32 of
+# 65; 12 tiles of LDS at 256 threads are capped at 8 (two per wave); 5 tiles leave waves 5-15 without one; 0 streams
+# every tile.
+@pytest.mark.parametrize("k,threads,blocks,tiles", [(2, 1024, 0, None), (3, 1024, 0, None), (2, 256, 0, 0),
+                                                    (3, 256, 2, 12), (2, 1024, 0, 5), (3, 1024, 0, 0)])
+def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dense_wide_graph, k, threads, blocks,
+                                           tiles):
     """The streaming wide kernels (16 bytes of vertex words per lane, one row dword per 4 or 2 vertices) on a dense
-    graph, where auto takes them for k = 3 too; each workgroup size, and fewer resident workgroups than slots, is
-    forced through a tuned-plan entry. The trajectory must be the oracle's."""
+    graph, where auto takes them for k = 3 too, with the slot's first 4 KiB tiles kept in LDS; each workgroup size,
+    fewer resident workgroups than slots, and fewer or no tiles in LDS are forced through a tuned-plan entry. The
+    trajectory must be the oracle's."""
     g = dense_wide_graph
     kw = dict(k=k, population=4, iterations=2, early_stop=2, seed=5, gww=0.25)
-    if threads == 1024 and blocks == 0:
+    if tiles is None:
         got, want = run_both(cga, orc, g, cpu_threads, **kw)
+        tiles = min(g.n // (1024 if k == 2 else 512), 39, threads // 64 * 2)
     else:
         descent = 3 if k == 2 else 4
         got, want = run_with_plan(cga, orc, g, cpu_threads, tmp_path,
-                                  f"{g.n} {k} 4 1 {descent} {threads} 0 0 0 {blocks}", **kw)
-    check_same(got, want, f"dense wide k={k} threads={threads} blocks={blocks}")
+                                  f"{g.n} {k} 4 1 {descent} {threads} 0 0 {tiles * 4096} {blocks}", **kw)
+    check_same(got, want, f"dense wide k={k} threads={threads} blocks={blocks} tiles in LDS={tiles}")
     assert (got["descent_name"], got["threads"]) == ("k2-wide" if k == 2 else "k3-wide", threads)
+    assert got["lds_bytes"] == tiles * 4096
 
 
 @pytest.mark.parametrize("n,p", [(40, 0.3), (64, 0.4), (65, 0.2), (257, 0.4), (1000, 0.05)])
