@@ -811,7 +811,8 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     size1 += (to == 1) - (from == 1);
     size2 += (to == 2) - (from == 2);
     // per-field increments of a neighbour, uniform; no field leaves [0, 2^15), so the halves never carry
-    const uint32_t dlo = static_cast<uint32_t>((to == 0) - (from == 0)) + (static_cast<uint32_t>((to == 1) - (from == 1)) << 16);
+    const uint32_t dlo =
+        static_cast<uint32_t>((to == 0) - (from == 0)) + (static_cast<uint32_t>((to == 1) - (from == 1)) << 16);
     const uint32_t dhi = static_cast<uint32_t>((to == 2) - (from == 2));
     const uint32_t bits = __builtin_amdgcn_alignbit(w1, w0, shift);
     asm volatile("" ::"v"(warm));
@@ -882,12 +883,13 @@ using Quad = uint32_t __attribute__((ext_vector_type(4)));
 // load j holds the kV consecutive vertices u = tile * kTile + (64 j + l) kV + i, whose row bits are bits
 // (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tile t belongs to wave t % kWaves (the
 // same owner as in ForEachOwned). Tiles t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile,
-// so they hold no vertex >= n, and resident <= kWaves * kResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its
-// resident tiles and the loads of its first streamed tile, then updates its resident tiles while those loads are in
-// flight, then streams the rest, loading its next tile before it consumes the current one (2 x 64 B of words and
-// two row dwords per lane in flight behind the arithmetic). Every word of a neighbour of the moved vertex gets `inc`,
-// the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of itself), a 16-byte group with
-// a changed word is written back, and visit(w, u) sees every vertex u < n in increasing order within the thread.
+// so they hold no vertex >= n, and resident <= kWaves * ResidentPerWave), the others in HBM (`st`). A wave first
+// issues the row dwords of its resident tiles and the loads of its first streamed tile, then updates its resident
+// tiles while those loads are in flight, then streams the rest, loading its next tile before it consumes the current
+// one (2 x 64 B of words and two row dwords per lane in flight behind the arithmetic). Every word of a neighbour of
+// the moved vertex gets `inc`, the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of
+// itself), a 16-byte group with a changed word is written back, and visit(w, u) sees every vertex u < n in
+// increasing order within the thread.
 // `st` is the slot's base (16-byte aligned, padded to whole tiles: the padding words take increments and are written
 // back, never visited). Row dwords of the last tile may lie beyond the row (the next row, or the zero tail after the
 // last row); their bits reach only padding words.
@@ -951,9 +953,10 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res,
       }
     }
   };
-  uint32_t rres[kResidentPerWave][kJ];
+  constexpr int kRes = ResidentPerWave(sizeof(Word));
+  uint32_t rres[kRes][kJ];
 #pragma unroll
-  for (int i = 0; i < kResidentPerWave; ++i) {
+  for (int i = 0; i < kRes; ++i) {
     if (wave + i * kWaves < resident) load_rows(wave + i * kWaves, rres[i]);
   }
   // this wave's first streamed tile: the smallest t >= resident with t % kWaves == wave
@@ -962,7 +965,7 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res,
   uint32_t cr[kJ], nr[kJ];
   if (t < tiles) load(t, cw, cr);
 #pragma unroll
-  for (int i = 0; i < kResidentPerWave; ++i) {
+  for (int i = 0; i < kRes; ++i) {
     const int tr = wave + i * kWaves;
     if (tr < resident) {
       Quad w[kJ];
@@ -1035,7 +1038,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   uint32_t* state = reinterpret_cast<uint32_t*>(r.work) + static_cast<size_t>(s) * WideStride(n, 4);
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / StreamTile(4), kWaves * kResidentPerWave));
+  const int resident = min(resident_tiles, min(n / StreamTile(4), kWaves * ResidentPerWave(4)));
   const int res_words = resident * StreamTile(4);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
@@ -1146,7 +1149,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
   uint64_t* state = r.work + static_cast<size_t>(s) * WideStride(n, 8);
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / StreamTile(8), kWaves * kResidentPerWave));
+  const int resident = min(resident_tiles, min(n / StreamTile(8), kWaves * ResidentPerWave(8)));
   const int res_words = resident * StreamTile(8);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint64_t> res = (LdsPtr<uint64_t>)dyn;
@@ -1887,7 +1890,7 @@ size_t WideResidentLdsBytes(int n, int k, int threads) {
   const int word_bytes = k == 2 ? 4 : 8;
   const int64_t tiles = n / StreamTile(word_bytes);  // whole tiles only
   const int64_t fit = static_cast<int64_t>(DescentLdsLimit() / kStreamTileBytes);
-  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * kResidentPerWave;
+  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * ResidentPerWave(word_bytes);
   return static_cast<size_t>(std::min({tiles, fit, per_waves})) * kStreamTileBytes;
 }
 

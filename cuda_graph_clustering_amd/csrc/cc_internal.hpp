@@ -102,9 +102,10 @@ __host__ __device__ constexpr int64_t WideStride(int n, int word_bytes) {
   return (static_cast<int64_t>(n) + StreamTile(word_bytes) - 1) / StreamTile(word_bytes) * StreamTile(word_bytes);
 }
 // A tile is 4 KiB at either word size. The first tiles of a slot may live in the workgroup's LDS instead of HBM (at
-// most kResidentPerWave per wave, as many as the LDS holds): they are never streamed.
+// most ResidentPerWave per wave, as many as the LDS holds): they are never streamed. A resident tile holds its row
+// dwords in registers through the move; three of them pushed the k = 2 kernel past 128 VGPRs at 1024 threads.
 constexpr int kStreamTileBytes = 64 * 16 * kStreamLoads;
-constexpr int kResidentPerWave = 2;
+__host__ __device__ constexpr int ResidentPerWave(int word_bytes) { return word_bytes == 8 ? 3 : 2; }
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 
@@ -184,7 +185,7 @@ size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
 size_t K3ChunkLdsBytes(int n);
 // LDS of the streaming wide-path descents (k = 2, k = 3) with `threads` per workgroup: the slot's first tiles, as
-// many as fit (at most kResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
+// many as fit (at most ResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
 // always streamed).
 size_t WideResidentLdsBytes(int n, int k, int threads);
 int RegistersVpt(int n, int k, int* threads);
