@@ -78,21 +78,22 @@ def make_graph(cga, gen):
 REGISTER_DESCENTS = ("registers", "k2-registers", "k3-packed", "generic-lds")
 
 
-def descent_bytes_per_move(n, k, descent, mean_degree):
+def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0):
     """Algorithmic global-memory bytes of one accepted move: the adjacency row of the moved vertex, plus what the
     kernel keeps in memory rather than on chip. The register kernels (and the generic kernel with G in LDS) hold
     the individual's state on chip, so a move reads only the row. The generic global kernel scans G and the labels
-    (k n |G| + n) in memory every move. The wide kernels stream one packed word per vertex (4 n for k = 2, 8 n for
-    k = 3) and write back the neighbours' words; the chunk-summary kernel re-reads only the 64-vertex chunks that
-    hold the moved vertex and its neighbours (at most mean degree + 1 chunks of 512 bytes, written back)."""
+    (k n |G| + n) in memory every move. The wide streaming kernels stream one packed word per vertex (4 n for k = 2,
+    8 n for k = 3) except the first tiles they keep in LDS (`lds_bytes`, the plan's dynamic LDS), and write back the
+    neighbours' words; the chunk-summary kernel re-reads only the 64-vertex chunks that hold the moved vertex and its
+    neighbours (at most mean degree + 1 chunks of 512 bytes, written back)."""
     words = max(1, (n + 63) // 64)
     row = 8 * words
     if descent in REGISTER_DESCENTS:
         return row
     if descent == "k2-wide":
-        return 4 * n + row + 4 * mean_degree
+        return 4 * n - lds_bytes + row + 4 * mean_degree
     if descent == "k3-wide":
-        return 8 * n + row + 8 * mean_degree
+        return 8 * n - lds_bytes + row + 8 * mean_degree
     if descent == "k3-chunk":
         return row + 2 * 512 * min(mean_degree + 1, words)
     gbytes = 2 if n <= 32767 else 4
@@ -543,11 +544,12 @@ def bench_scale(cga, args):
                 later_s = (runs[3][0] - first_s) / 2
                 moves = (runs[3][1]["accepted_moves"] - first["accepted_moves"]) / (2 * common["population"])
                 # all slots descend at once: algorithmic bytes of one move of every slot per move time, against HBM
-                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n)
+                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n, first["lds_bytes"])
                 gbs = common["population"] * per_move / (later_s / moves) / 1e9
                 point = {"n": n, "density": density, "edges": g.edges, "k": k, "descent": first["descent_name"],
                          "iteration_s": round(later_s, 3), "first_iteration_s": round(first_s, 3),
                          "moves_per_descent": round(moves), "us_per_move": round(1e6 * later_s / moves, 2),
+                         "lds_bytes": first["lds_bytes"],
                          "hbm": {"bytes_per_move": round(per_move), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 3)},
                          "f_after_3": runs[3][1]["objective"], "f_recounted": True}
