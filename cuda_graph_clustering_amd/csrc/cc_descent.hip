@@ -197,7 +197,7 @@ __device__ __forceinline__ void DescentGenericBody(const RunDev& r, int s) {
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
-        ctl->error = 1;
+        ctl->error = kErrorMoveCap;
         ctl->stopped = 1;
       }
       break;
@@ -354,7 +354,7 @@ __device__ __forceinline__ void DescentK2Body(const RunDev& r, int s) {
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
-        ctl->error = 1;
+        ctl->error = kErrorMoveCap;
         ctl->stopped = 1;
       }
       break;
@@ -661,7 +661,7 @@ __device__ __forceinline__ void DescentRegsBody(const RunDev& r, int s) {
     }
   }
   if (capped && tid == 0) {
-    ctl->error = 1;
+    ctl->error = kErrorMoveCap;
     ctl->stopped = 1;
   }
 
@@ -848,7 +848,7 @@ __device__ __forceinline__ void DescentK3PackedBody(const RunDev& r, int s) {
     }
   }
   if (capped && tid == 0) {
-    ctl->error = 1;
+    ctl->error = kErrorMoveCap;
     ctl->stopped = 1;
   }
 
@@ -1099,7 +1099,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
     }
   }
   if (capped && tid == 0) {
-    ctl->error = 1;
+    ctl->error = kErrorMoveCap;
     ctl->stopped = 1;
   }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
@@ -1225,7 +1225,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
     }
   }
   if (capped && tid == 0) {
-    ctl->error = 1;
+    ctl->error = kErrorMoveCap;
     ctl->stopped = 1;
   }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
@@ -1465,7 +1465,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
     objective += delta;
     if (++applied >= cap) {
       if (tid == 0) {
-        ctl->error = 1;
+        ctl->error = kErrorMoveCap;
         ctl->stopped = 1;
       }
       break;
@@ -1554,9 +1554,18 @@ __global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, Ite
   Persistent(runs, list, [resident_tiles](const RunDev& r, int s) { DescentK3WideBody<T>(r, s, resident_tiles); });
 }
 
+// The launch passes the dynamic LDS it allocated: an item whose table and lists would not fit stops its run with an
+// error instead of reaching beyond the allocation.
 template <int T>
-__global__ void __launch_bounds__(T) KernelDescentK3Chunk(const RunDev* runs, ItemList list) {
-  Persistent(runs, list, [](const RunDev& r, int s) { DescentK3ChunkBody<T>(r, s); });
+__global__ void __launch_bounds__(T) KernelDescentK3Chunk(const RunDev* runs, ItemList list, int lds_bytes) {
+  Persistent(runs, list, [lds_bytes](const RunDev& r, int s) {
+    if (K3ChunkLdsBytes(r.n) <= static_cast<size_t>(lds_bytes)) {
+      DescentK3ChunkBody<T>(r, s);
+    } else if (threadIdx.x == 0) {
+      r.ctl->error = kErrorLds;
+      r.ctl->stopped = 1;
+    }
+  });
 }
 
 // The mixed descent: one persistent launch for a batch whose runs use different register kernels. The queue holds
@@ -1661,6 +1670,16 @@ struct Go {
     if (mode == GoMode::kProbe) return true;
     AllowLds(kernel, lds);
     if (mode == GoMode::kLaunch) kernel<<<Blocks(), threads, lds, stream>>>(runs, *list);
+    return true;
+  }
+
+  // A kernel that takes the plan's dynamic LDS size as its last argument.
+  template <typename Kernel>
+  bool RunWithLds(Kernel kernel, int threads) const {
+    if (mode == GoMode::kProbe) return true;
+    const size_t lds = plan->lds_bytes;
+    AllowLds(kernel, lds);
+    if (mode == GoMode::kLaunch) kernel<<<Blocks(), threads, lds, stream>>>(runs, *list, static_cast<int>(lds));
     return true;
   }
 };
@@ -1850,8 +1869,8 @@ bool Dispatch(const Go& go) {
       return !p.wide && p.k == 3 && p.threads == 1024 && GoK3Packed(go);
     case Descent::kK3Chunk:
       if (!p.wide || p.k != 3 || p.lds_bytes == 0) return false;
-      if (p.threads == 256) return go.Run(KernelDescentK3Chunk<256>, 256, p.lds_bytes);
-      if (p.threads == 1024) return go.Run(KernelDescentK3Chunk<1024>, 1024, p.lds_bytes);
+      if (p.threads == 256) return go.RunWithLds(KernelDescentK3Chunk<256>, 256);
+      if (p.threads == 1024) return go.RunWithLds(KernelDescentK3Chunk<1024>, 1024);
       return false;
     case Descent::kMixed:
       if (p.wide) return false;
@@ -1894,10 +1913,6 @@ size_t WideResidentLdsBytes(int n, int k, int threads) {
   return static_cast<size_t>(std::min({tiles, fit, per_waves})) * kStreamTileBytes;
 }
 
-size_t K3ChunkLdsBytes(int n) {
-  const size_t chunks = (static_cast<size_t>(n) + 63) / 64;
-  return (kPairs3 + 2) * chunks * sizeof(uint32_t);  // pair table + two dirty lists
-}
 
 int MixedVariant(int n, int k, DescentPlan* plan) {
   auto pick = [&](const MixedDesc* table, int count, int threads) {

@@ -33,7 +33,7 @@ struct Ctl {
   int32_t stopped;            // set once the run is over: every later kernel skips the run
   int32_t have_record;
   int32_t rec_slot;
-  int32_t error;              // a descent exceeded MoveCap(): the run is stopped and reported as failed
+  int32_t error;              // kErrorMoveCap / kErrorLds: the run is stopped and reported as failed
   int64_t rec_f;
   int64_t local_searches;
   int64_t accepted_moves;
@@ -92,6 +92,13 @@ struct ItemList {
 // therefore only be reached through a broken invariant (a wrong delta or G): the descent then stops, sets
 // ctl->error, and the run fails instead of spinning on the device. No valid descent is cut short.
 __host__ __device__ inline int64_t MoveCap(int64_t f_start) { return f_start + 1; }
+
+// Ctl::error: why a descent stopped its run on the device.
+constexpr int32_t kErrorMoveCap = 1;  // it reached MoveCap()
+constexpr int32_t kErrorLds = 2;      // its launch allocated less LDS than its kernel needs
+inline const char* DeviceErrorText(int32_t error) {
+  return error == kErrorLds ? "a descent was launched with less LDS than it needs" : "a descent exceeded its move cap";
+}
 
 // The streaming wide-path descents (k = 2: 4-byte vertex words, k = 3: 8-byte words) read and write a slot's words
 // 16 bytes per lane. A wave's tile is kStreamLoads such loads of its 64 lanes; each slot's words start 16-byte
@@ -183,7 +190,9 @@ void SetDescentAttributes(const DescentPlan& plan);
 bool DescentSupported(const DescentPlan& plan);
 size_t DescentLdsLimit();
 size_t GenericLdsBytes(int n, int k, int wide);
-size_t K3ChunkLdsBytes(int n);
+// LDS of the k = 3 chunk-summary descent for n vertices (64-vertex chunks): the pair table (6 x u32 per chunk) and
+// the dirty-chunk list of each move parity (2 x u32 per chunk).
+__host__ __device__ constexpr size_t K3ChunkLdsBytes(int n) { return (static_cast<size_t>(n) + 63) / 64 * (6 + 2) * 4; }
 // LDS of the streaming wide-path descents (k = 2, k = 3) with `threads` per workgroup: the slot's first tiles, as
 // many as fit (at most ResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
 // always streamed).

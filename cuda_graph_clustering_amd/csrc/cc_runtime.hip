@@ -829,7 +829,7 @@ void SolvePbils(const DeviceGraph& g, const cc_pbils_params& p, int32_t* labels_
   span.End(stream.get(), &stats->iteration_ms, &spans);
   const Ctl c = batch.PullCtl()[0];
   span.Collect();
-  if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
+  if (c.error) Fail(CC_HIP_EDEVICE, "%s; the run was stopped", DeviceErrorText(c.error));
   batch.PullLabels(0, labels_out);
   batch.PullTrace(0, c.iterations_done, trace_out);
   FillStats(c, spec, labels_out, stats);
@@ -883,7 +883,7 @@ void SolvePbilsStatic(const DeviceGraph& g, const cc_pbils_params& p, hipStream_
   batch.Start();
   RunToCompletion(batch, {spec}, p.verbose != 0, false, p.time_limit_sec, nullptr, stats, started);
   const Ctl c = batch.PullCtl()[0];
-  if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
+  if (c.error) Fail(CC_HIP_EDEVICE, "%s; the run was stopped", DeviceErrorText(c.error));
   batch.PullLabels(0, labels_out);
   batch.PullTrace(0, c.iterations_done, trace_out);
   FillStats(c, spec, labels_out, stats);
@@ -969,7 +969,7 @@ void SolveNeighborhood(const DeviceGraph& g, const cc_pbils_params& params, bool
   }
   const Ctl c = batch.PullCtl()[0];
   if (t) t->Collect();
-  if (c.error) Fail(CC_HIP_EDEVICE, "a descent exceeded its move cap; the run was stopped");
+  if (c.error) Fail(CC_HIP_EDEVICE, "%s; the run was stopped", DeviceErrorText(c.error));
   batch.PullLabels(0, labels_out);
   FillStats(c, spec, labels_out, stats);
   stats->iterations_done = 0;
@@ -1008,7 +1008,7 @@ void SolveBatch(std::vector<BatchRun>& runs, bool use_graph) {
   const Ctl* ctl = batch.PullCtl();
   std::vector<Ctl> copy(ctl, ctl + specs.size());
   for (size_t i = 0; i < runs.size(); ++i) {
-    if (copy[i].error) Fail(CC_HIP_EDEVICE, "run %zu: a descent exceeded its move cap", i);
+    if (copy[i].error) Fail(CC_HIP_EDEVICE, "run %zu: %s", i, DeviceErrorText(copy[i].error));
     batch.PullLabels(static_cast<int>(i), runs[i].labels);
     batch.PullTrace(static_cast<int>(i), copy[i].iterations_done, runs[i].trace);
     FillStats(copy[i], specs[i], runs[i].labels, runs[i].stats);

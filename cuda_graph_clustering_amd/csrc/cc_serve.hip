@@ -307,7 +307,7 @@ class Server {
           const uint32_t n = a.request.n;
           if (ctl[i].error) {
             r.code = CC_HIP_EDEVICE;
-            r.error = "a descent exceeded its move cap; the run was stopped";
+            r.error = std::string(DeviceErrorText(ctl[i].error)) + "; the run was stopped";
           } else {
             r.labels.resize(n);
             set->PullLabels(static_cast<int>(i), r.labels.data());

@@ -238,7 +238,8 @@ int LoadTuned(const char* path) {
     p.blocks = blocks;
     // any single-run kernel (the mixed batch kernel is chosen per batch, not per shape)
     const bool valid = descent >= 0 && descent <= static_cast<int>(Descent::kK3Packed) &&
-                       p.descent != Descent::kMixed && lds <= DescentLdsLimit() && DescentSupported(p);
+                       p.descent != Descent::kMixed && lds <= DescentLdsLimit() && DescentSupported(p) &&
+                       (p.descent != Descent::kK3Chunk || lds >= K3ChunkLdsBytes(static_cast<int>(n)));
     if (!valid) {
       std::fclose(f);
       Fail(CC_HIP_EINVAL, "%s: entry %d names no supported descent plan", path, line);

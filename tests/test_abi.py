@@ -90,9 +90,15 @@ def test_tune_table_load_validates_entries(tmp_path):
     bad.write_text("2000 5 16 0 99 1024 2 0 0 0\n")
     garbled = tmp_path / "garbled.txt"
     garbled.write_text("2000 5 16 0 5 1024 two\n")
+    # the k = 3 chunk kernel at n = 1000 (16 chunks) needs 32 bytes of LDS per chunk: an entry with less is refused
+    chunk_ok = tmp_path / "chunk_ok.txt"
+    chunk_ok.write_text("1000 3 6 1 7 1024 0 0 512 0\n")
+    chunk_small = tmp_path / "chunk_small.txt"
+    chunk_small.write_text("1000 3 6 1 7 1024 0 0 511 0\n")
     try:
         assert cga.tune_load(good) == 1
-        for path in (bad, garbled):
+        assert cga.tune_load(chunk_ok) == 1
+        for path in (bad, garbled, chunk_small):
             with pytest.raises(cga.CcError) as e:
                 cga.tune_load(path)
             assert e.value.code == cga.EINVAL
