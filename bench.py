@@ -82,8 +82,8 @@ def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0):
     """Algorithmic global-memory bytes of one accepted move: the adjacency row of the moved vertex, plus what the
     kernel keeps in memory rather than on chip. The register kernels (and the generic kernel with G in LDS) hold
     the individual's state on chip, so a move reads only the row. The generic global kernel scans G and the labels
-    (k n |G| + n) in memory every move. The wide streaming kernels stream one packed word per vertex (4 n for k = 2,
-    8 n for k = 3) except the first tiles they keep in LDS (`lds_bytes`, the plan's dynamic LDS), and write back the
+    (k n |G| + n) in memory every move. The wide streaming kernels stream one packed word per vertex (3 n bytes for
+    k = 2, 8 n for k = 3) except the first tiles they keep in LDS (`lds_bytes`, the plan's dynamic LDS), and write back the
     neighbours' words; the chunk-summary kernel re-reads only the 64-vertex chunks that hold the moved vertex and its
     neighbours (at most mean degree + 1 chunks of 512 bytes, written back)."""
     words = max(1, (n + 63) // 64)
@@ -91,7 +91,7 @@ def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0):
     if descent in REGISTER_DESCENTS:
         return row
     if descent == "k2-wide":
-        return 4 * n - lds_bytes + row + 4 * mean_degree
+        return 3 * n - lds_bytes + row + 3 * mean_degree
     if descent == "k3-wide":
         return 8 * n - lds_bytes + row + 8 * mean_degree
     if descent == "k3-chunk":

@@ -878,29 +878,156 @@ using LdsPtr = __attribute__((address_space(3))) Word*;
 // 16 bytes of vertex words, one global_load_dwordx4 / global_store_dwordx4 (ds_read_b128 / ds_write_b128) per lane.
 using Quad = uint32_t __attribute__((ext_vector_type(4)));
 
-// One pass of a streaming wide-path descent over a slot's packed vertex words (Word: uint32_t at k = 2, uint64_t at
-// k = 3) after a move, 16 bytes per lane. With kV = 16 / sizeof(Word), a wave's tile is kStreamLoads loads: lane l of
-// load j holds the kV consecutive vertices u = tile * kTile + (64 j + l) kV + i, whose row bits are bits
-// (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tile t belongs to wave t % kWaves (the
-// same owner as in ForEachOwned). Tiles t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile,
-// so they hold no vertex >= n, and resident <= kWaves * ResidentPerWave), the others in HBM (`st`). A wave first
-// issues the row dwords of its resident tiles and the loads of its first streamed tile, then updates its resident
-// tiles while those loads are in flight, then streams the rest, loading its next tile before it consumes the current
-// one (2 x 64 B of words and two row dwords per lane in flight behind the arithmetic). Every word of a neighbour of
-// the moved vertex gets `inc`, the moved vertex's own word becomes relabel(word) (its label; it is no neighbour of
-// itself), a 16-byte group with a changed word is written back, and visit(w, u) sees every vertex u < n in
-// increasing order within the thread.
-// `st` is the slot's base (16-byte aligned, padded to whole tiles: the padding words take increments and are written
+// k = 3 streaming words: G0 | G1 << 20 | G2 << 40 | label << 62 (G < 2^20). A neighbour update is one 64-bit add
+// of (1 << 20 to) - (1 << 20 from): G[from] >= 1 there and G[to] + 1 <= deg, so no field borrows or carries. Best
+// target of a vertex with label a: the smaller h_c = s_c - 2 G_c of the two other clusters, ties to the smaller c;
+// delta = h_c - h_a + 1 (the reference's cuda/cc_local_search_k3.cuh:20-67 states the same algebra).
+constexpr int kK3Bits = 20;
+constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
+constexpr int kK3LabelShift = 62;
+
+// The group of vertex words that a lane of a streaming descent loads and stores at once: kV consecutive vertices in
+// kBytes bytes of the slot's buffer (group g at byte kBytes * g).
+//
+// k = 3: two 64-bit words in 16 bytes.
+struct GroupK3 {
+  static constexpr int kV = 2;
+  static constexpr int kBytes = 16;
+  using Word = uint64_t;
+  using Inc = uint64_t;
+  struct Regs {
+    Quad q;
+  };
+  __device__ static Regs Load(GlobalPtr<uint32_t> p, size_t g) { return {*(GlobalPtr<Quad>)(p + 4 * g)}; }
+  __device__ static Regs Load(LdsPtr<uint32_t> p, size_t g) { return {*(LdsPtr<Quad>)(p + 4 * g)}; }
+  __device__ static void Store(GlobalPtr<uint32_t> p, size_t g, const Regs& r) { *(GlobalPtr<Quad>)(p + 4 * g) = r.q; }
+  __device__ static void Store(LdsPtr<uint32_t> p, size_t g, const Regs& r) { *(LdsPtr<Quad>)(p + 4 * g) = r.q; }
+  template <int I>
+  __device__ static Word Get(const Regs& r) {
+    return static_cast<uint64_t>(r.q[2 * I]) | static_cast<uint64_t>(r.q[2 * I + 1]) << 32;
+  }
+  template <int I>
+  __device__ static void Set(Regs& r, Word w) {
+    r.q[2 * I] = static_cast<uint32_t>(w);
+    r.q[2 * I + 1] = static_cast<uint32_t>(w >> 32);
+  }
+  // inc added to the words whose bit i is set in `bits` (bits < 4)
+  __device__ static void Add(Regs& r, uint32_t bits, Inc inc) {
+    Set<0>(r, Get<0>(r) + (inc & (Word(0) - (bits & 1u))));
+    Set<1>(r, Get<1>(r) + (inc & (Word(0) - ((bits >> 1) & 1u))));
+  }
+  __device__ static Regs Pack(const Word* w) {
+    Regs r;
+    Set<0>(r, w[0]);
+    Set<1>(r, w[1]);
+    return r;
+  }
+};
+
+// k = 2: four 24-bit fields f = 2 e + label + 2^23 (e = G1 - G0) in 12 bytes (one global_load_dwordx3), the 96-bit
+// little-endian value f0 + f1 2^24 + f2 2^48 + f3 2^72. |2 e + label| <= 2 deg + 1 < 2^21 for n <= 2^20, so every
+// field stays inside (0, 2^24) and adding to one field never carries into the next: a group's neighbour update is
+// one 96-bit add of the fields' increments.
+constexpr uint32_t kK2Bias = 1u << 23;
+struct GroupK2 {
+  static constexpr int kV = 4;
+  static constexpr int kBytes = 12;
+  using Word = uint32_t;
+  using Inc = int;  // +4 (a neighbour of a vertex moving 0 -> 1: e + 2) or -4
+  struct Regs {
+    uint32_t d0, d1, d2;
+  };
+  __device__ static Regs Load(GlobalPtr<uint32_t> p, size_t g) {
+    p += 3 * g;
+    return {p[0], p[1], p[2]};
+  }
+  __device__ static Regs Load(LdsPtr<uint32_t> p, size_t g) {
+    p += 3 * g;
+    return {p[0], p[1], p[2]};
+  }
+  __device__ static void Store(GlobalPtr<uint32_t> p, size_t g, const Regs& r) {
+    p += 3 * g;
+    p[0] = r.d0;
+    p[1] = r.d1;
+    p[2] = r.d2;
+  }
+  __device__ static void Store(LdsPtr<uint32_t> p, size_t g, const Regs& r) {
+    p += 3 * g;
+    p[0] = r.d0;
+    p[1] = r.d1;
+    p[2] = r.d2;
+  }
+  template <int I>
+  __device__ static Word Get(const Regs& r) {
+    if constexpr (I == 0) return r.d0 & 0xFFFFFFu;
+    if constexpr (I == 1) return __builtin_amdgcn_alignbit(r.d1, r.d0, 24) & 0xFFFFFFu;
+    if constexpr (I == 2) return __builtin_amdgcn_alignbit(r.d2, r.d1, 16) & 0xFFFFFFu;
+    return r.d2 >> 8;
+  }
+  __device__ static void Add(Regs& r, uint32_t bits, Inc inc) {
+    // 4 in field i: bit 2 + 24 i of the 96-bit value
+    const uint32_t a0 = ((bits & 1u) << 2) | ((bits & 2u) << 25);
+    const uint32_t a1 = (bits & 4u) << 16;
+    const uint32_t a2 = (bits & 8u) << 7;
+    unsigned __int128 v = static_cast<unsigned __int128>(r.d0) | static_cast<unsigned __int128>(r.d1) << 32 |
+                          static_cast<unsigned __int128>(r.d2) << 64;
+    const unsigned __int128 a = static_cast<unsigned __int128>(a0) | static_cast<unsigned __int128>(a1) << 32 |
+                                static_cast<unsigned __int128>(a2) << 64;
+    v = inc > 0 ? v + a : v - a;
+    r.d0 = static_cast<uint32_t>(v);
+    r.d1 = static_cast<uint32_t>(v >> 32);
+    r.d2 = static_cast<uint32_t>(v >> 64);
+  }
+  // the label bit of field i
+  __device__ static void FlipLabel(Regs& r, int i) {
+    if (i == 0) r.d0 ^= 1u;
+    if (i == 1) r.d0 ^= 1u << 24;
+    if (i == 2) r.d1 ^= 1u << 16;
+    if (i == 3) r.d2 ^= 1u << 8;
+  }
+  __device__ static Regs Pack(const Word* f) {
+    return {f[0] | f[1] << 24, f[1] >> 8 | f[2] << 16, f[2] >> 16 | f[3] << 8};
+  }
+};
+
+template <typename G, typename F, int... I>
+__device__ __forceinline__ void ForEachWordImpl(const typename G::Regs& r, F& f, std::integer_sequence<int, I...>) {
+  (f(G::template Get<I>(r), I), ...);
+}
+// f(word, i) for the words i = 0 .. kV - 1 of a group, in order
+template <typename G, typename F>
+__device__ __forceinline__ void ForEachWord(const typename G::Regs& r, F&& f) {
+  ForEachWordImpl<G>(r, f, std::make_integer_sequence<int, G::kV>{});
+}
+
+// A wave's tile of the streaming layout: kStreamLoads groups per lane, lane l of load j holding group
+// tile * kTile / kV + 64 j + l, i.e. the kV consecutive vertices u = tile * kTile + (64 j + l) kV + i. Tile t
+// belongs to wave t % kWaves.
+template <typename G>
+constexpr int kGroupTile = kWave * G::kV * kStreamLoads;
+
+// One pass of a streaming wide-path descent over a slot's vertex words after a move (G: GroupK2 or GroupK3). The row
+// bits of a lane's group are bits (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tiles
+// t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile, so they hold no vertex >= n, and
+// resident <= kWaves * ResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its resident
+// tiles and the loads of its first streamed tile, then updates its resident tiles while those loads are in flight,
+// then streams the rest, loading its next tile before it consumes the current one (two tiles of groups and row dwords
+// per lane in flight behind the arithmetic). Every word of a neighbour of the moved vertex gets `inc`,
+// relabel(group, i) gives the moved vertex (word i of its group) its new label (it is no neighbour of itself), a
+// group with a changed word is written back, and visit(word, u) sees every vertex u < n in increasing order within
+// the thread. `st` is the slot's base (padded to whole tiles: the padding words take increments and are written
 // back, never visited). Row dwords of the last tile may lie beyond the row (the next row, or the zero tail after the
 // last row); their bits reach only padding words.
-template <int T, typename Word, typename Relabel, typename Visit>
-__device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res, int resident, GlobalWords32 row,
-                                           int n, int wave, int lane, Word inc, int mover, Relabel&& relabel,
-                                           Visit&& visit) {
-  constexpr int kV = 16 / static_cast<int>(sizeof(Word));
+template <int T, typename G, typename Relabel, typename Visit>
+__device__ __forceinline__ void StreamMove(GlobalPtr<uint32_t> st, LdsPtr<uint32_t> res, int resident,
+                                           GlobalWords32 row, int n, int wave, int lane, typename G::Inc inc,
+                                           int mover, Relabel&& relabel, Visit&& visit) {
+  using Regs = typename G::Regs;
+  constexpr int kV = G::kV;
   constexpr int kJ = kStreamLoads;
-  constexpr int kTile = StreamTile(sizeof(Word));
+  constexpr int kTile = kGroupTile<G>;
   constexpr int kWaves = T / kWave;
+  constexpr int kRes = ResidentPerWave(kV == 4 ? 2 : 3);
   const int tiles = (n + kTile - 1) / kTile;
   const uint32_t lane_bit = static_cast<uint32_t>(lane * kV) & 31u;
   const int lane_dword = (lane * kV) >> 5;
@@ -909,51 +1036,42 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res,
   const int mover_load = (mover - mover_tile * kTile) / (kWave * kV);
   const int mover_lane = (mover / kV) % kWave;
   const int mover_word = mover % kV;
-  auto quad = [&](int t, int j) { return static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane; };
+  auto group = [&](int t, int j) { return static_cast<size_t>(t) * (kTile / kV) + j * kWave + lane; };
   auto load_rows = [&](int t, uint32_t* rr) {
 #pragma unroll
     for (int j = 0; j < kJ; ++j) rr[j] = row[((t * kTile + j * kWave * kV) >> 5) + lane_dword];
   };
-  auto load = [&](int t, Quad* w, uint32_t* rr) {
+  auto load = [&](int t, Regs* w, uint32_t* rr) {
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
-      w[j] = st[quad(t, j)];
+      w[j] = G::Load(st, group(t, j));
       rr[j] = row[((t * kTile + j * kWave * kV) >> 5) + lane_dword];
     }
   };
-  auto consume = [&](int t, const Quad* w, const uint32_t* rr, auto full, auto in_lds) {
+  auto consume = [&](int t, const Regs* w, const uint32_t* rr, auto full, auto in_lds) {
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
-      const uint32_t bits = rr[j] >> lane_bit;
-      Word ws[kV];
-      __builtin_memcpy(ws, &w[j], 16);
-#pragma unroll
-      for (int i = 0; i < kV; ++i) ws[i] += inc & (Word(0) - static_cast<Word>((bits >> i) & 1u));
-      bool changed = (bits & ((1u << kV) - 1u)) != 0;
+      const uint32_t bits = (rr[j] >> lane_bit) & ((1u << kV) - 1u);
+      Regs x = w[j];
+      G::Add(x, bits, inc);
+      bool changed = bits != 0;
       if (t == mover_tile && j == mover_load && lane == mover_lane) {  // once per move, in one lane
-#pragma unroll
-        for (int i = 0; i < kV; ++i) {
-          if (i == mover_word) ws[i] = relabel(ws[i]);
-        }
+        relabel(x, mover_word);
         changed = true;
       }
       if (changed) {
-        Quad out;
-        __builtin_memcpy(&out, ws, 16);
         if constexpr (decltype(in_lds)::value) {
-          res[quad(t, j)] = out;
+          G::Store(res, group(t, j), x);
         } else {
-          st[quad(t, j)] = out;
+          G::Store(st, group(t, j), x);
         }
       }
       const int u0 = t * kTile + (j * kWave + lane) * kV;
-#pragma unroll
-      for (int i = 0; i < kV; ++i) {
-        if (decltype(full)::value || u0 + i < n) visit(ws[i], u0 + i);
-      }
+      ForEachWord<G>(x, [&](typename G::Word word, int i) {
+        if (decltype(full)::value || u0 + i < n) visit(word, u0 + i);
+      });
     }
   };
-  constexpr int kRes = ResidentPerWave(sizeof(Word));
   uint32_t rres[kRes][kJ];
 #pragma unroll
   for (int i = 0; i < kRes; ++i) {
@@ -961,16 +1079,16 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res,
   }
   // this wave's first streamed tile: the smallest t >= resident with t % kWaves == wave
   int t = wave + kWaves * ((max(resident - wave, 0) + kWaves - 1) / kWaves);
-  Quad cw[kJ], nw[kJ];
+  Regs cw[kJ], nw[kJ];
   uint32_t cr[kJ], nr[kJ];
   if (t < tiles) load(t, cw, cr);
 #pragma unroll
   for (int i = 0; i < kRes; ++i) {
     const int tr = wave + i * kWaves;
     if (tr < resident) {
-      Quad w[kJ];
+      Regs w[kJ];
 #pragma unroll
-      for (int j = 0; j < kJ; ++j) w[j] = res[quad(tr, j)];
+      for (int j = 0; j < kJ; ++j) w[j] = G::Load(res, group(tr, j));
       consume(tr, w, rres[i], std::true_type{}, std::true_type{});
     }
   }
@@ -989,38 +1107,27 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<Quad> st, LdsPtr<Quad> res,
   }
 }
 
-// The vertices u < n of a slot's streaming state that this thread owns (StreamMove's layout), in increasing order:
-// visit(u).
-template <int T, int kWordBytes, typename Visit>
-__device__ __forceinline__ void ForEachOwned(int n, int wave, int lane, Visit&& visit) {
-  constexpr int kV = 16 / kWordBytes;
-  constexpr int kTile = StreamTile(kWordBytes);
+// The groups of a slot's streaming state that this thread owns (StreamMove's layout), in increasing order:
+// visit(g, u0) with the group's first vertex u0 < n (its other vertices may lie beyond n).
+template <int T, typename G, typename Visit>
+__device__ __forceinline__ void ForEachOwnedGroup(int n, int wave, int lane, Visit&& visit) {
+  constexpr int kTile = kGroupTile<G>;
   for (int t = wave; t * kTile < n; t += T / kWave) {
 #pragma unroll
     for (int j = 0; j < kStreamLoads; ++j) {
-      const int u0 = t * kTile + (j * kWave + lane) * kV;
-#pragma unroll
-      for (int i = 0; i < kV; ++i) {
-        if (u0 + i < n) visit(u0 + i);
-      }
+      const int u0 = t * kTile + (j * kWave + lane) * G::kV;
+      if (u0 < n) visit(static_cast<size_t>(u0 / G::kV), u0);
     }
   }
 }
 
-// The thread that owns vertex v in StreamMove's layout.
-template <int T, int kWordBytes>
-__device__ __forceinline__ int StreamOwner(int v) {
-  constexpr int kTile = StreamTile(kWordBytes);
-  const int t = v / kTile;
-  return (t % (T / kWave)) * kWave + ((v - t * kTile) / (16 / kWordBytes)) % kWave;
-}
-
-// Wide path, k = 2: one 32-bit word per vertex, w = 2 e + label with e = G1 - G0, in the run's work buffer
-// (reference algebra cuda/cc_local_search_k2.cuh:251-254): delta = sv - w for label 0 and w + 1 - sv for label 1; a
-// neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored from w and the degrees at the
-// end.
+// Wide path, k = 2: one 24-bit field per vertex (GroupK2), f = 2 e + label + 2^23 with e = G1 - G0, in the run's
+// work buffer (reference algebra cuda/cc_local_search_k2.cuh:251-254, w = f - 2^23): delta = sv - w for label 0 and
+// w + 1 - sv for label 1; a neighbour of a vertex moving 0 -> 1 gains 4 (e + 2), 1 -> 0 loses 4. G is restored from
+// w and the degrees at the end.
 template <int T>
 __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int resident_tiles) {
+  using G = GroupK2;
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1036,36 +1143,48 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   const int src = src_sh;
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 2 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
-  uint32_t* state = reinterpret_cast<uint32_t*>(r.work) + static_cast<size_t>(s) * WideStride(n, 4);
+  const GlobalPtr<uint32_t> st =
+      (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 2));
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / StreamTile(4), kWaves * ResidentPerWave(4)));
-  const int res_words = resident * StreamTile(4);
+  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(2)));
+  const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
   int n0 = r.pop.sizes[static_cast<size_t>(src) * 2];
   const int64_t f_start = r.pop.f[src];
 
-  int sv = n - 2 * n0 + 1;
-  int best_d = 0, best_u = -1, best_w = 0;
-  auto consider = [&](uint32_t word, int u) {
-    const int w = static_cast<int>(word);
-    const int d = (w & 1) ? w + 1 - sv : sv - w;
+  int sv_bias = n - 2 * n0 + 1 + static_cast<int>(kK2Bias);  // sv + 2^23
+  int best_d = 0, best_u = -1;
+  uint32_t best_f = 0;
+  auto consider = [&](uint32_t f, int u) {
+    const int x = static_cast<int>(f);
+    const int d = (f & 1u) ? x + 1 - sv_bias : sv_bias - x;
     if (d < best_d) {
       best_d = d;
       best_u = u;
-      best_w = w;
+      best_f = f;
     }
   };
-  ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
-    const uint32_t w = static_cast<uint32_t>(2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1));
-    if (u < res_words) {
-      res[u] = w;
-    } else {
-      state[u] = w;
+  auto key_of = [&]() {
+    return best_u < 0 ? kNoMove
+                      : PackMove(best_d, best_u, static_cast<int>(best_f & 1u), static_cast<int>(best_f & 1u) ^ 1);
+  };
+  ForEachOwnedGroup<T, G>(n, wave, lane, [&](size_t g, int u0) {
+    uint32_t f[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int u = u0 + i;
+      f[i] = u < n ? static_cast<uint32_t>(2 * (g_in[n + u] - g_in[u]) + (lab_in[u] & 1) + static_cast<int>(kK2Bias))
+                   : kK2Bias;
+      if (u < n) consider(f[i], u);
     }
-    consider(w, u);
+    if (g < res_groups) {
+      G::Store(res, g, G::Pack(f));
+    } else {
+      G::Store(st, g, G::Pack(f));
+    }
   });
-  uint64_t key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
+  uint64_t key = key_of();
   int64_t objective = f_start;
   int64_t applied = 0;
   const int64_t cap = MoveCap(f_start);
@@ -1073,7 +1192,6 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
-  const GlobalPtr<Quad> st = (GlobalPtr<Quad>)state;
   bool capped = false;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
@@ -1084,14 +1202,13 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
     const int v = static_cast<int>(mv >> 12);
     const int from = static_cast<int>((mv >> 6) & 63u);
     n0 += from ? 1 : -1;
-    sv = n - 2 * n0 + 1;
-    const uint32_t inc = from ? static_cast<uint32_t>(-4) : 4u;
+    sv_bias = n - 2 * n0 + 1 + static_cast<int>(kK2Bias);
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_u = -1;
-    StreamMove<T, uint32_t>(st, (LdsPtr<Quad>)res, resident, row, n, wave, lane, inc, v,
-                            [](uint32_t w) { return w ^ 1u; }, consider);
-    key = best_u < 0 ? kNoMove : PackMove(best_d, best_u, best_w & 1, (best_w & 1) ^ 1);
+    StreamMove<T, G>(st, res, resident, row, n, wave, lane, from ? -4 : 4, v,
+                     [](G::Regs& x, int i) { G::FlipLabel(x, i); }, consider);
+    key = key_of();
     objective += delta;
     if (++applied >= cap) {
       capped = true;
@@ -1104,13 +1221,19 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 2 * n;
-  ForEachOwned<T, 4>(n, wave, lane, [&](int u) {
-    const int w = static_cast<int>(u < res_words ? res[u] : state[u]);
-    const int e = w >> 1;
-    const int degree = r.deg[u];
-    lab_out[u] = static_cast<uint8_t>(w & 1);
-    g_out[u] = (degree - e) / 2;
-    g_out[n + u] = (degree + e) / 2;
+  ForEachOwnedGroup<T, G>(n, wave, lane, [&](size_t g, int u0) {
+    const G::Regs x = g < res_groups ? G::Load(res, g) : G::Load(st, g);
+    ForEachWord<G>(x, [&](uint32_t f, int i) {
+      const int u = u0 + i;
+      if (u < n) {
+        const int w = static_cast<int>(f) - static_cast<int>(kK2Bias);
+        const int e = w >> 1;
+        const int degree = r.deg[u];
+        lab_out[u] = static_cast<uint8_t>(w & 1);
+        g_out[u] = (degree - e) / 2;
+        g_out[n + u] = (degree + e) / 2;
+      }
+    });
   });
   if (tid == 0) {
     r.next.sizes[static_cast<size_t>(s) * 2] = n0;
@@ -1120,17 +1243,10 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   }
 }
 
-// Wide path, k = 3: one 64-bit word per vertex, G0 | G1 << 20 | G2 << 40 | label << 62 (G < 2^20), in the run's
-// work buffer. A neighbour update is one 64-bit add of (1 << 20 to) - (1 << 20 from): G[from] >= 1 there and
-// G[to] + 1 <= deg, so no field borrows or carries. Best target of a vertex with label a: the smaller
-// h_c = s_c - 2 G_c of the two other clusters, ties to the smaller c; delta = h_c - h_a + 1 (the reference's
-// cuda/cc_local_search_k3.cuh:20-67 states the same algebra).
-constexpr int kK3Bits = 20;
-constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
-constexpr int kK3LabelShift = 62;
-
+// Wide path, k = 3: one 64-bit word per vertex (GroupK3) in the run's work buffer.
 template <int T>
 __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int resident_tiles) {
+  using G = GroupK3;
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1147,12 +1263,13 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
-  uint64_t* state = r.work + static_cast<size_t>(s) * WideStride(n, 8);
+  const GlobalPtr<uint32_t> st =
+      (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 3));
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / StreamTile(8), kWaves * ResidentPerWave(8)));
-  const int res_words = resident * StreamTile(8);
+  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
+  const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
-  const LdsPtr<uint64_t> res = (LdsPtr<uint64_t>)dyn;
+  const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
   int s0 = sizes_in[0], s1 = sizes_in[1], s2 = sizes_in[2];
   const int64_t f_start = r.pop.f[src];
 
@@ -1176,16 +1293,22 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   auto key_of = [&]() {
     return best_move < 0 ? kNoMove : PackMove(best_d, best_move >> 4, (best_move >> 2) & 3, best_move & 3);
   };
-  ForEachOwned<T, 8>(n, wave, lane, [&](int u) {
-    const uint64_t w = static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
-                       static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
-                       static_cast<uint64_t>(lab_in[u]) << kK3LabelShift;
-    if (u < res_words) {
-      res[u] = w;
-    } else {
-      state[u] = w;
+  ForEachOwnedGroup<T, G>(n, wave, lane, [&](size_t g, int u0) {
+    uint64_t w[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int u = u0 + i;
+      w[i] = u < n ? static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
+                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
+                         static_cast<uint64_t>(lab_in[u]) << kK3LabelShift
+                   : 0;
+      if (u < n) consider(w[i], u);
     }
-    consider(w, u);
+    if (g < res_groups) {
+      G::Store(res, g, G::Pack(w));
+    } else {
+      G::Store(st, g, G::Pack(w));
+    }
   });
   uint64_t key = key_of();
   int64_t objective = f_start;
@@ -1195,7 +1318,6 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const uint64_t* const row_base =
       reinterpret_cast<const uint64_t*>(Uniform(static_cast<int64_t>(reinterpret_cast<uintptr_t>(r.bits))));
   const int row_words = __builtin_amdgcn_readfirstlane(r.words);
-  const GlobalPtr<Quad> st = (GlobalPtr<Quad>)state;
   bool capped = false;
   while (true) {
     const uint64_t best = BlockMin<kWaves>(key, partial[parity]);
@@ -1214,9 +1336,17 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_move = -1;
-    StreamMove<T, uint64_t>(
-        st, (LdsPtr<Quad>)res, resident, row, n, wave, lane, inc, v,
-        [label_to](uint64_t w) { return (w & ~(3ull << kK3LabelShift)) | label_to; }, consider);
+    StreamMove<T, G>(
+        st, res, resident, row, n, wave, lane, inc, v,
+        [label_to](G::Regs& x, int i) {
+          auto relabel = [label_to](uint64_t w) { return (w & ~(3ull << kK3LabelShift)) | label_to; };
+          if (i == 0) {
+            G::Set<0>(x, relabel(G::Get<0>(x)));
+          } else {
+            G::Set<1>(x, relabel(G::Get<1>(x)));
+          }
+        },
+        consider);
     key = key_of();
     objective += delta;
     if (++applied >= cap) {
@@ -1230,12 +1360,17 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   }
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
-  ForEachOwned<T, 8>(n, wave, lane, [&](int u) {
-    const uint64_t w = u < res_words ? res[u] : state[u];
-    g_out[u] = static_cast<int32_t>(w & kK3Mask);
-    g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
-    g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
-    lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
+  ForEachOwnedGroup<T, G>(n, wave, lane, [&](size_t g, int u0) {
+    const G::Regs x = g < res_groups ? G::Load(res, g) : G::Load(st, g);
+    ForEachWord<G>(x, [&](uint64_t w, int i) {
+      const int u = u0 + i;
+      if (u < n) {
+        g_out[u] = static_cast<int32_t>(w & kK3Mask);
+        g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
+        g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
+        lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
+      }
+    });
   });
   if (tid == 0) {
     int32_t* sizes = r.next.sizes + static_cast<size_t>(s) * 3;
@@ -1294,7 +1429,7 @@ __device__ __forceinline__ void DescentK3ChunkBody(const RunDev& r, int s) {
   const int32_t* g_in = static_cast<const int32_t*>(r.pop.g) + static_cast<size_t>(src) * 3 * n;
   const uint8_t* lab_in = r.pop.labels + static_cast<size_t>(src) * n;
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
-  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * WideStride(n, 8));
+  const GlobalPtr<uint64_t> st = (GlobalPtr<uint64_t>)(r.work + static_cast<size_t>(s) * WideStride(n, 3));
   // the three cluster sizes (< 2^21) as 21-bit fields of one uniform word: a move updates them with scalar adds
   uint64_t szp = static_cast<uint64_t>(sizes_in[0]) | static_cast<uint64_t>(sizes_in[1]) << 21 |
                  static_cast<uint64_t>(sizes_in[2]) << 42;
@@ -1819,7 +1954,7 @@ bool GoRegs(const Go& go) {
 template <template <int> class Wide>
 bool GoWide(const Go& go) {
   const size_t lds = go.plan->lds_bytes;
-  const int resident = static_cast<int>(lds / kStreamTileBytes);
+  const int resident = static_cast<int>(lds / WideTileBytes(Wide<256>::kK));
   auto run = [&](auto kernel, int threads) {
     if (go.mode == GoMode::kProbe) return true;
     AllowLds(kernel, lds);
@@ -1844,10 +1979,12 @@ bool GoK3Packed(const Go& go) {
 
 template <int T>
 struct WideK2 {
+  static constexpr int kK = 2;
   static constexpr auto kernel = KernelDescentK2Wide<T>;
 };
 template <int T>
 struct WideK3 {
+  static constexpr int kK = 3;
   static constexpr auto kernel = KernelDescentK3Wide<T>;
 };
 
@@ -1906,11 +2043,10 @@ size_t GenericLdsBytes(int n, int k, int wide) {
 }
 
 size_t WideResidentLdsBytes(int n, int k, int threads) {
-  const int word_bytes = k == 2 ? 4 : 8;
-  const int64_t tiles = n / StreamTile(word_bytes);  // whole tiles only
-  const int64_t fit = static_cast<int64_t>(DescentLdsLimit() / kStreamTileBytes);
-  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * ResidentPerWave(word_bytes);
-  return static_cast<size_t>(std::min({tiles, fit, per_waves})) * kStreamTileBytes;
+  const int64_t tiles = n / WideTileVertices(k);  // whole tiles only
+  const int64_t fit = static_cast<int64_t>(DescentLdsLimit() / WideTileBytes(k));
+  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * ResidentPerWave(k);
+  return static_cast<size_t>(std::min({tiles, fit, per_waves})) * WideTileBytes(k);
 }
 
 

@@ -64,7 +64,7 @@ struct RunDev {
   uint64_t* masks;       // [P][k][words] rebuild scratch
   int64_t* intra;        // [P] rebuild scratch: sum_v G[label v][v]
   uint64_t* work;        // packed vertex words of the wide descents (null for other plans): k = 3 chunk kernel
-                         // [P][n] u64; streaming kernels [P][WideStride(n, word)] of u32 (k = 2) or u64 (k = 3)
+                         // [P][WideStride(n, 3)] u64; streaming kernels [P][WideSlotBytes(n, k)] bytes
   uint64_t init_stream, select_stream, perturb_stream;
   int32_t tournament, copies, early_stop, iterations;
   float perturbation;
@@ -100,19 +100,26 @@ inline const char* DeviceErrorText(int32_t error) {
   return error == kErrorLds ? "a descent was launched with less LDS than it needs" : "a descent exceeded its move cap";
 }
 
-// The streaming wide-path descents (k = 2: 4-byte vertex words, k = 3: 8-byte words) read and write a slot's words
-// 16 bytes per lane. A wave's tile is kStreamLoads such loads of its 64 lanes; each slot's words start 16-byte
-// aligned and are padded to whole tiles (WideStride words per slot in RunDev::work).
+// The streaming wide-path descents load and store a group of consecutive vertices per lane: k = 2 four 24-bit fields
+// in 12 bytes, k = 3 two 64-bit words in 16 bytes. A wave's tile is kStreamLoads such groups of its 64 lanes (1024
+// vertices in 3 KiB at k = 2, 512 in 4 KiB at k = 3); each slot is padded to whole tiles (WideSlotBytes per slot in
+// RunDev::work, 16-byte aligned).
 constexpr int kStreamLoads = 4;
-__host__ __device__ constexpr int StreamTile(int word_bytes) { return 64 * (16 / word_bytes) * kStreamLoads; }
-__host__ __device__ constexpr int64_t WideStride(int n, int word_bytes) {
-  return (static_cast<int64_t>(n) + StreamTile(word_bytes) - 1) / StreamTile(word_bytes) * StreamTile(word_bytes);
+__host__ __device__ constexpr int WideGroupVertices(int k) { return k == 2 ? 4 : 2; }
+__host__ __device__ constexpr int WideGroupBytes(int k) { return k == 2 ? 12 : 16; }
+__host__ __device__ constexpr int WideTileVertices(int k) { return 64 * WideGroupVertices(k) * kStreamLoads; }
+__host__ __device__ constexpr int WideTileBytes(int k) { return 64 * WideGroupBytes(k) * kStreamLoads; }
+// n rounded up to whole tiles
+__host__ __device__ constexpr int64_t WideStride(int n, int k) {
+  return (static_cast<int64_t>(n) + WideTileVertices(k) - 1) / WideTileVertices(k) * WideTileVertices(k);
 }
-// A tile is 4 KiB at either word size. The first tiles of a slot may live in the workgroup's LDS instead of HBM (at
-// most ResidentPerWave per wave, as many as the LDS holds): they are never streamed. A resident tile holds its row
-// dwords in registers through the move; three of them pushed the k = 2 kernel past 128 VGPRs at 1024 threads.
-constexpr int kStreamTileBytes = 64 * 16 * kStreamLoads;
-__host__ __device__ constexpr int ResidentPerWave(int word_bytes) { return word_bytes == 8 ? 3 : 2; }
+__host__ __device__ constexpr int64_t WideSlotBytes(int n, int k) {
+  return WideStride(n, k) / WideGroupVertices(k) * WideGroupBytes(k);
+}
+// The first tiles of a slot may live in the workgroup's LDS instead of HBM (at most ResidentPerWave per wave, as
+// many as the LDS holds): they are never streamed. A resident tile holds its row dwords in registers through the
+// move; three of them pushed the k = 2 kernel past 128 VGPRs at 1024 threads.
+__host__ __device__ constexpr int ResidentPerWave(int k) { return k == 3 ? 3 : 2; }
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 

@@ -307,22 +307,25 @@ def dense_wide_graph(cga):
                                                     (3, 256, 2, 16), (2, 1024, 0, 5), (3, 1024, 0, 0)])
 def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dense_wide_graph, k, threads, blocks,
                                            tiles):
-    """The streaming wide kernels (16 bytes of vertex words per lane, one row dword per 4 or 2 vertices) on a dense
-    graph, where auto takes them for k = 3 too, with the slot's first 4 KiB tiles kept in LDS; each workgroup size,
-    fewer resident workgroups than slots, and fewer or no tiles in LDS are forced through a tuned-plan entry. The
-    trajectory must be the oracle's."""
+    """The streaming wide kernels (per lane a group of four 24-bit vertex fields in 12 bytes at k = 2, two 64-bit
+    words in 16 bytes at k = 3, one row dword per group) on a dense graph, where auto takes them for k = 3 too, with
+    the slot's first tiles (3 KiB at k = 2, 4 KiB at k = 3) kept in LDS; each workgroup size, fewer resident
+    workgroups than slots, and fewer or no tiles in LDS are forced through a tuned-plan entry. The trajectory must
+    be the oracle's."""
     g = dense_wide_graph
+    tile_bytes = 3072 if k == 2 else 4096
     kw = dict(k=k, population=4, iterations=2, early_stop=2, seed=5, gww=0.25)
     if tiles is None:
         got, want = run_both(cga, orc, g, cpu_threads, **kw)
-        tiles = min(g.n // (1024 if k == 2 else 512), 39, threads // 64 * (2 if k == 2 else 3))
+        tiles = min(g.n // (1024 if k == 2 else 512), (160 * 1024 - 2048) // tile_bytes,
+                    threads // 64 * (2 if k == 2 else 3))
     else:
         descent = 3 if k == 2 else 4
         got, want = run_with_plan(cga, orc, g, cpu_threads, tmp_path,
-                                  f"{g.n} {k} 4 1 {descent} {threads} 0 0 {tiles * 4096} {blocks}", **kw)
+                                  f"{g.n} {k} 4 1 {descent} {threads} 0 0 {tiles * tile_bytes} {blocks}", **kw)
     check_same(got, want, f"dense wide k={k} threads={threads} blocks={blocks} tiles in LDS={tiles}")
     assert (got["descent_name"], got["threads"]) == ("k2-wide" if k == 2 else "k3-wide", threads)
-    assert got["lds_bytes"] == tiles * 4096
+    assert got["lds_bytes"] == tiles * tile_bytes
 
 
 @pytest.mark.parametrize("n,p", [(40, 0.3), (64, 0.4), (65, 0.2), (257, 0.4), (1000, 0.05)])
