@@ -893,6 +893,8 @@ constexpr int kK3LabelShift = 62;
 struct GroupK3 {
   static constexpr int kV = 2;
   static constexpr int kBytes = 16;
+  static constexpr int kBits = kK3Bits;
+  static constexpr int kLabelShift = kK3LabelShift;
   using Word = uint64_t;
   using Inc = uint64_t;
   struct Regs {
@@ -987,6 +989,53 @@ struct GroupK2 {
   }
   __device__ static Regs Pack(const Word* f) {
     return {f[0] | f[1] << 24, f[1] >> 8 | f[2] << 16, f[2] >> 16 | f[3] << 8};
+  }
+};
+
+// k = 3 when every degree is below 2^15: 48-bit words G0 | G1 << 15 | G2 << 30 | label << 45, two in 12 bytes (the
+// 96-bit little-endian value w0 + w1 2^48). A neighbour update adds (1 << 15 to) - (1 << 15 from) to its word as a
+// signed 96-bit add; as with the 64-bit words no field borrows or carries (G[from] >= 1, G[to] + 1 <= deg < 2^15).
+struct GroupK3N {
+  static constexpr int kV = 2;
+  static constexpr int kBytes = 12;
+  static constexpr int kBits = 15;
+  static constexpr int kLabelShift = 45;
+  using Word = uint64_t;
+  using Inc = uint64_t;  // (1 << 15 to) - (1 << 15 from), two's complement
+  using Regs = GroupK2::Regs;
+  __device__ static Regs Load(GlobalPtr<uint32_t> p, size_t g) { return GroupK2::Load(p, g); }
+  __device__ static Regs Load(LdsPtr<uint32_t> p, size_t g) { return GroupK2::Load(p, g); }
+  __device__ static void Store(GlobalPtr<uint32_t> p, size_t g, const Regs& r) { GroupK2::Store(p, g, r); }
+  __device__ static void Store(LdsPtr<uint32_t> p, size_t g, const Regs& r) { GroupK2::Store(p, g, r); }
+  template <int I>
+  __device__ static Word Get(const Regs& r) {
+    if constexpr (I == 0) return static_cast<uint64_t>(r.d1 & 0xFFFFu) << 32 | r.d0;
+    return static_cast<uint64_t>(r.d2) << 16 | r.d1 >> 16;
+  }
+  template <int I>
+  __device__ static void Set(Regs& r, Word w) {
+    if constexpr (I == 0) {
+      r.d0 = static_cast<uint32_t>(w);
+      r.d1 = (r.d1 & 0xFFFF0000u) | (static_cast<uint32_t>(w >> 32) & 0xFFFFu);
+    } else {
+      r.d1 = (r.d1 & 0xFFFFu) | static_cast<uint32_t>(w) << 16;
+      r.d2 = static_cast<uint32_t>(w >> 16);
+    }
+  }
+  __device__ static void Add(Regs& r, uint32_t bits, Inc inc) {
+    const __int128 i = static_cast<int64_t>(inc);
+    const __int128 a = ((bits & 1u) ? i : __int128(0)) + ((bits & 2u) ? i * (__int128(1) << 48) : __int128(0));
+    unsigned __int128 v = static_cast<unsigned __int128>(r.d0) | static_cast<unsigned __int128>(r.d1) << 32 |
+                          static_cast<unsigned __int128>(r.d2) << 64;
+    v += static_cast<unsigned __int128>(a);
+    r.d0 = static_cast<uint32_t>(v);
+    r.d1 = static_cast<uint32_t>(v >> 32);
+    r.d2 = static_cast<uint32_t>(v >> 64);
+  }
+  __device__ static Regs Pack(const Word* w) {
+    return {static_cast<uint32_t>(w[0]),
+            (static_cast<uint32_t>(w[0] >> 32) & 0xFFFFu) | static_cast<uint32_t>(w[1]) << 16,
+            static_cast<uint32_t>(w[1] >> 16)};
   }
 };
 
@@ -1243,10 +1292,13 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   }
 }
 
-// Wide path, k = 3: one 64-bit word per vertex (GroupK3) in the run's work buffer.
-template <int T>
+// Wide path, k = 3: one packed word per vertex (G: GroupK3, or GroupK3N when every degree is below 2^15) in the
+// run's work buffer, which is sized for GroupK3. `resident_tiles` counts GroupK3 tiles of the launch's LDS.
+template <int T, typename G>
 __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int resident_tiles) {
-  using G = GroupK3;
+  constexpr int kBits = G::kBits;
+  constexpr uint64_t kMask = (1ull << kBits) - 1;
+  constexpr int kLabelShift = G::kLabelShift;
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1265,8 +1317,10 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
   const GlobalPtr<uint32_t> st =
       (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 3));
-  // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
+  // the slot's first whole tiles in LDS (the launch provides resident_tiles GroupK3 tiles of it)
+  constexpr int kTileBytes = kWave * G::kBytes * kStreamLoads;
+  const int lds_tiles = resident_tiles * WideTileBytes(3) / kTileBytes;
+  const int resident = min(lds_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
   const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
@@ -1275,10 +1329,10 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
 
   int best_d = 0, best_move = -1;
   auto consider = [&](uint64_t w, int u) {
-    const int a = static_cast<int>(w >> kK3LabelShift);
-    const int h0 = s0 - 2 * static_cast<int>(w & kK3Mask);
-    const int h1 = s1 - 2 * static_cast<int>((w >> kK3Bits) & kK3Mask);
-    const int h2 = s2 - 2 * static_cast<int>((w >> (2 * kK3Bits)) & kK3Mask);
+    const int a = static_cast<int>(w >> kLabelShift);
+    const int h0 = s0 - 2 * static_cast<int>(w & kMask);
+    const int h1 = s1 - 2 * static_cast<int>((w >> kBits) & kMask);
+    const int h2 = s2 - 2 * static_cast<int>((w >> (2 * kBits)) & kMask);
     const int ha = a == 0 ? h0 : (a == 1 ? h1 : h2);
     const int lo = a == 0 ? h1 : h0;  // the other cluster with the smaller index
     const int hi = a == 2 ? h1 : h2;
@@ -1298,9 +1352,9 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int u = u0 + i;
-      w[i] = u < n ? static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kK3Bits |
-                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kK3Bits) |
-                         static_cast<uint64_t>(lab_in[u]) << kK3LabelShift
+      w[i] = u < n ? static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kBits |
+                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kBits) |
+                         static_cast<uint64_t>(lab_in[u]) << kLabelShift
                    : 0;
       if (u < n) consider(w[i], u);
     }
@@ -1331,19 +1385,19 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
     s0 += (to == 0) - (from == 0);
     s1 += (to == 1) - (from == 1);
     s2 += (to == 2) - (from == 2);
-    const uint64_t inc = (1ull << (kK3Bits * to)) - (1ull << (kK3Bits * from));
-    const uint64_t label_to = static_cast<uint64_t>(to) << kK3LabelShift;
+    const uint64_t inc = (1ull << (kBits * to)) - (1ull << (kBits * from));
+    const uint64_t label_to = static_cast<uint64_t>(to) << kLabelShift;
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_move = -1;
     StreamMove<T, G>(
         st, res, resident, row, n, wave, lane, inc, v,
-        [label_to](G::Regs& x, int i) {
-          auto relabel = [label_to](uint64_t w) { return (w & ~(3ull << kK3LabelShift)) | label_to; };
+        [label_to](typename G::Regs& x, int i) {
+          auto relabel = [label_to](uint64_t w) { return (w & ~(3ull << kLabelShift)) | label_to; };
           if (i == 0) {
-            G::Set<0>(x, relabel(G::Get<0>(x)));
+            G::template Set<0>(x, relabel(G::template Get<0>(x)));
           } else {
-            G::Set<1>(x, relabel(G::Get<1>(x)));
+            G::template Set<1>(x, relabel(G::template Get<1>(x)));
           }
         },
         consider);
@@ -1361,14 +1415,14 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   uint8_t* lab_out = r.next.labels + static_cast<size_t>(s) * n;
   int32_t* g_out = static_cast<int32_t*>(r.next.g) + static_cast<size_t>(s) * 3 * n;
   ForEachOwnedGroup<T, G>(n, wave, lane, [&](size_t g, int u0) {
-    const G::Regs x = g < res_groups ? G::Load(res, g) : G::Load(st, g);
+    const typename G::Regs x = g < res_groups ? G::Load(res, g) : G::Load(st, g);
     ForEachWord<G>(x, [&](uint64_t w, int i) {
       const int u = u0 + i;
       if (u < n) {
-        g_out[u] = static_cast<int32_t>(w & kK3Mask);
-        g_out[n + u] = static_cast<int32_t>((w >> kK3Bits) & kK3Mask);
-        g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kK3Bits)) & kK3Mask);
-        lab_out[u] = static_cast<uint8_t>(w >> kK3LabelShift);
+        g_out[u] = static_cast<int32_t>(w & kMask);
+        g_out[n + u] = static_cast<int32_t>((w >> kBits) & kMask);
+        g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kBits)) & kMask);
+        lab_out[u] = static_cast<uint8_t>(w >> kLabelShift);
       }
     });
   });
@@ -1684,9 +1738,29 @@ __global__ void __launch_bounds__(T) KernelDescentK2Wide(const RunDev* runs, Ite
   Persistent(runs, list, [resident_tiles](const RunDev& r, int s) { DescentK2WideBody<T>(r, s, resident_tiles); });
 }
 
+// The largest degree of a run's graph, in every thread of the workgroup.
+template <int T>
+__device__ __forceinline__ int BlockMaxDegree(const RunDev& r) {
+  __shared__ int32_t max_sh;
+  if (threadIdx.x == 0) max_sh = 0;
+  __syncthreads();
+  int m = 0;
+  for (int u = threadIdx.x; u < r.n; u += T) m = max(m, r.deg[u]);
+  m = static_cast<int>(~WaveMin(~static_cast<uint32_t>(m)));
+  if ((threadIdx.x & 63) == 0) atomicMax(&max_sh, m);
+  __syncthreads();
+  return max_sh;
+}
+
 template <int T>
 __global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list, int resident_tiles) {
-  Persistent(runs, list, [resident_tiles](const RunDev& r, int s) { DescentK3WideBody<T>(r, s, resident_tiles); });
+  Persistent(runs, list, [resident_tiles](const RunDev& r, int s) {
+    if (BlockMaxDegree<T>(r) < (1 << GroupK3N::kBits)) {
+      DescentK3WideBody<T, GroupK3N>(r, s, resident_tiles);
+    } else {
+      DescentK3WideBody<T, GroupK3>(r, s, resident_tiles);
+    }
+  });
 }
 
 // The launch passes the dynamic LDS it allocated: an item whose table and lists would not fit stops its run with an

@@ -328,6 +328,29 @@ def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dens
     assert got["lds_bytes"] == tiles * tile_bytes
 
 
+def test_k3_streaming_words_follow_the_largest_degree(cga, orc, cpu_threads):
+    """The k = 3 streaming kernel packs 48-bit words (15-bit counts, two vertices in 12 bytes) while every degree is
+    below 2^15 and 64-bit words otherwise. An ER graph (n = 40000, p = 0.05: dense enough for the streaming kernel)
+    takes the 48-bit words; the same graph with vertex 0 joined to every other vertex (degree 39999) takes the 64-bit
+    words. Both must follow the oracle."""
+    base = cga.erdos_renyi(40000, 0.05, 13)
+    n = base.n
+    bits = base.bits.copy()
+    row0 = np.full(bits.shape[1], np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    row0[0] &= ~np.uint64(1)
+    if n % 64:
+        row0[-1] &= np.uint64((1 << (n % 64)) - 1)
+    bits[0] = row0
+    bits[1:, 0] |= np.uint64(1)
+    hub = cga.from_bits(bits, n)
+    assert hub.edges == base.edges + (n - 1) - int(np.unpackbits(base.bits[0].view(np.uint8)).sum())
+    kw = dict(k=3, population=4, iterations=2, early_stop=2, seed=9, gww=0.25)
+    for name, g in (("max degree < 2^15", base), ("hub of degree n - 1", hub)):
+        got, want = run_both(cga, orc, g, cpu_threads, **kw)
+        check_same(got, want, f"k3 streaming, {name}")
+        assert got["descent_name"] == "k3-wide", name
+
+
 @pytest.mark.parametrize("n,p", [(40, 0.3), (64, 0.4), (65, 0.2), (257, 0.4), (1000, 0.05)])
 def test_chunk_kernel_small_graphs(cga, orc, cpu_threads, tmp_path, n, p):
     """The k = 3 chunk-summary kernel below and at one chunk (n = 40, 64), one vertex into a second chunk (65) and on
