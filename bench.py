@@ -78,22 +78,36 @@ def make_graph(cga, gen):
 REGISTER_DESCENTS = ("registers", "k2-registers", "k3-packed", "generic-lds")
 
 
-def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0):
+def streamed_state_bytes(n, k, lds_bytes, max_degree):
+    """State bytes a streaming wide descent passes over per move: one packed word per vertex (k = 2: 3 bytes; k = 3:
+    6 bytes while every degree is below 2^15, else 8) less the whole tiles it keeps in LDS (k = 2: 3 KiB tiles of 1024
+    vertices, at most two per wave; k = 3: the plan's LDS counts 4 KiB tiles of 512 64-bit words, which hold
+    4/3 as many 3 KiB tiles of 48-bit words, at most three per wave of 1024 threads)."""
+    if k == 2:
+        tiles = min(lds_bytes // 3072, n // 1024, 32)
+        return 3 * n - 3072 * tiles
+    if max_degree < 1 << 15:
+        tiles = min(lds_bytes // 4096 * 4096 // 3072, n // 512, 48)
+        return 6 * n - 3072 * tiles
+    tiles = min(lds_bytes // 4096, n // 512, 48)
+    return 8 * n - 4096 * tiles
+
+
+def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0, max_degree=0):
     """Algorithmic global-memory bytes of one accepted move: the adjacency row of the moved vertex, plus what the
     kernel keeps in memory rather than on chip. The register kernels (and the generic kernel with G in LDS) hold
     the individual's state on chip, so a move reads only the row. The generic global kernel scans G and the labels
-    (k n |G| + n) in memory every move. The wide streaming kernels stream one packed word per vertex (3 n bytes for
-    k = 2, 8 n for k = 3) except the first tiles they keep in LDS (`lds_bytes`, the plan's dynamic LDS), and write back the
+    (k n |G| + n) in memory every move. The wide streaming kernels stream one packed word per vertex except the first
+    tiles they keep in LDS (streamed_state_bytes; `lds_bytes` is the plan's dynamic LDS), and write back the
     neighbours' words; the chunk-summary kernel re-reads only the 64-vertex chunks that hold the moved vertex and its
     neighbours (at most mean degree + 1 chunks of 512 bytes, written back)."""
     words = max(1, (n + 63) // 64)
     row = 8 * words
     if descent in REGISTER_DESCENTS:
         return row
-    if descent == "k2-wide":
-        return 3 * n - lds_bytes + row + 3 * mean_degree
-    if descent == "k3-wide":
-        return 8 * n - lds_bytes + row + 8 * mean_degree
+    if descent in ("k2-wide", "k3-wide"):
+        state = streamed_state_bytes(n, k, lds_bytes, max_degree)
+        return state + row + state / n * mean_degree
     if descent == "k3-chunk":
         return row + 2 * 512 * min(mean_degree + 1, words)
     gbytes = 2 if n <= 32767 else 4
@@ -523,13 +537,17 @@ def bench_scale(cga, args):
     timed by a solve of one iteration, the later ones by the difference to a solve of three. Each result's f must
     equal the library's own recount of its labels. The per-move figure divides an iteration by the accepted moves of
     one descent (all slots descend at once), which compares graphs whose descents differ in length. One JSON line."""
+    import numpy as np
+
     sizes = [(int(a), float(b)) for a, b in (s.split(":") for s in args.scale_sizes.split(","))]
     common = dict(population=128, tournament=5, perturbation=0.4, gww=0.1, seed=5)
     points = []
     for n, density in sizes:
         t0 = time.perf_counter()
         g = cga.erdos_renyi(n, density, 7)
-        note(f"scale: ER n={n} p={density} built in {time.perf_counter() - t0:.1f} s ({g.edges} edges)")
+        max_degree = int(np.bitwise_count(g.bits).sum(axis=1, dtype=np.int64).max())
+        note(f"scale: ER n={n} p={density} built in {time.perf_counter() - t0:.1f} s ({g.edges} edges, largest "
+             f"degree {max_degree})")
         with cga.DeviceGraph(g) as dg:
             for k in (int(x) for x in args.scale_ks.split(",")):
                 runs = {}
@@ -544,12 +562,13 @@ def bench_scale(cga, args):
                 later_s = (runs[3][0] - first_s) / 2
                 moves = (runs[3][1]["accepted_moves"] - first["accepted_moves"]) / (2 * common["population"])
                 # all slots descend at once: algorithmic bytes of one move of every slot per move time, against HBM
-                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n, first["lds_bytes"])
+                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n, first["lds_bytes"],
+                                                  max_degree)
                 gbs = common["population"] * per_move / (later_s / moves) / 1e9
                 point = {"n": n, "density": density, "edges": g.edges, "k": k, "descent": first["descent_name"],
                          "iteration_s": round(later_s, 3), "first_iteration_s": round(first_s, 3),
                          "moves_per_descent": round(moves), "us_per_move": round(1e6 * later_s / moves, 2),
-                         "lds_bytes": first["lds_bytes"],
+                         "lds_bytes": first["lds_bytes"], "largest_degree": max_degree,
                          "hbm": {"bytes_per_move": round(per_move), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 3)},
                          "f_after_3": runs[3][1]["objective"], "f_recounted": True}
