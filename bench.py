@@ -78,22 +78,16 @@ def make_graph(cga, gen):
 REGISTER_DESCENTS = ("registers", "k2-registers", "k3-packed", "generic-lds")
 
 
-def streamed_state_bytes(n, k, lds_bytes, max_degree):
+def streamed_state_bytes(n, k, lds_bytes):
     """State bytes a streaming wide descent passes over per move: one packed word per vertex (k = 2: 3 bytes; k = 3:
-    6 bytes while every degree is below 2^15, else 8) less the whole tiles it keeps in LDS (k = 2: 3 KiB tiles of 1024
-    vertices, at most two per wave; k = 3: the plan's LDS counts 4 KiB tiles of 512 64-bit words, which hold
-    4/3 as many 3 KiB tiles of 48-bit words, at most three per wave of 1024 threads)."""
+    6 bytes) less the whole 3 KiB tiles it keeps in LDS (1024 vertices at k = 2, at most two per wave; 512 at k = 3,
+    at most three per wave; 1024 threads)."""
     if k == 2:
-        tiles = min(lds_bytes // 3072, n // 1024, 32)
-        return 3 * n - 3072 * tiles
-    if max_degree < 1 << 15:
-        tiles = min(lds_bytes // 4096 * 4096 // 3072, n // 512, 48)
-        return 6 * n - 3072 * tiles
-    tiles = min(lds_bytes // 4096, n // 512, 48)
-    return 8 * n - 4096 * tiles
+        return 3 * n - 3072 * min(lds_bytes // 3072, n // 1024, 32)
+    return 6 * n - 3072 * min(lds_bytes // 3072, n // 512, 48)
 
 
-def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0, max_degree=0):
+def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0):
     """Algorithmic global-memory bytes of one accepted move: the adjacency row of the moved vertex, plus what the
     kernel keeps in memory rather than on chip. The register kernels (and the generic kernel with G in LDS) hold
     the individual's state on chip, so a move reads only the row. The generic global kernel scans G and the labels
@@ -106,7 +100,7 @@ def descent_bytes_per_move(n, k, descent, mean_degree, lds_bytes=0, max_degree=0
     if descent in REGISTER_DESCENTS:
         return row
     if descent in ("k2-wide", "k3-wide"):
-        state = streamed_state_bytes(n, k, lds_bytes, max_degree)
+        state = streamed_state_bytes(n, k, lds_bytes)
         return state + row + state / n * mean_degree
     if descent == "k3-chunk":
         return row + 2 * 512 * min(mean_degree + 1, words)
@@ -562,8 +556,7 @@ def bench_scale(cga, args):
                 later_s = (runs[3][0] - first_s) / 2
                 moves = (runs[3][1]["accepted_moves"] - first["accepted_moves"]) / (2 * common["population"])
                 # all slots descend at once: algorithmic bytes of one move of every slot per move time, against HBM
-                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n, first["lds_bytes"],
-                                                  max_degree)
+                per_move = descent_bytes_per_move(n, k, first["descent_name"], 2.0 * g.edges / n, first["lds_bytes"])
                 gbs = common["population"] * per_move / (later_s / moves) / 1e9
                 point = {"n": n, "density": density, "edges": g.edges, "k": k, "descent": first["descent_name"],
                          "iteration_s": round(later_s, 3), "first_iteration_s": round(first_s, 3),

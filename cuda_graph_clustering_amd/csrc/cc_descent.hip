@@ -878,10 +878,11 @@ using LdsPtr = __attribute__((address_space(3))) Word*;
 // 16 bytes of vertex words, one global_load_dwordx4 / global_store_dwordx4 (ds_read_b128 / ds_write_b128) per lane.
 using Quad = uint32_t __attribute__((ext_vector_type(4)));
 
-// k = 3 streaming words: G0 | G1 << 20 | G2 << 40 | label << 62 (G < 2^20). A neighbour update is one 64-bit add
-// of (1 << 20 to) - (1 << 20 from): G[from] >= 1 there and G[to] + 1 <= deg, so no field borrows or carries. Best
-// target of a vertex with label a: the smaller h_c = s_c - 2 G_c of the two other clusters, ties to the smaller c;
-// delta = h_c - h_a + 1 (the reference's cuda/cc_local_search_k3.cuh:20-67 states the same algebra).
+// k = 3 64-bit vertex words of the chunk-summary descent: G0 | G1 << 20 | G2 << 40 | label << 62 (G < 2^20). A
+// neighbour update is one 64-bit add of (1 << 20 to) - (1 << 20 from): G[from] >= 1 there and G[to] + 1 <= deg, so no
+// field borrows or carries. Best target of a vertex with label a: the smaller h_c = s_c - 2 G_c of the two other
+// clusters, ties to the smaller c; delta = h_c - h_a + 1 (the reference's cuda/cc_local_search_k3.cuh:20-67 states
+// the same algebra).
 constexpr int kK3Bits = 20;
 constexpr uint64_t kK3Mask = (1ull << kK3Bits) - 1;
 constexpr int kK3LabelShift = 62;
@@ -889,43 +890,6 @@ constexpr int kK3LabelShift = 62;
 // The group of vertex words that a lane of a streaming descent loads and stores at once: kV consecutive vertices in
 // kBytes bytes of the slot's buffer (group g at byte kBytes * g).
 //
-// k = 3: two 64-bit words in 16 bytes.
-struct GroupK3 {
-  static constexpr int kV = 2;
-  static constexpr int kBytes = 16;
-  static constexpr int kBits = kK3Bits;
-  static constexpr int kLabelShift = kK3LabelShift;
-  using Word = uint64_t;
-  using Inc = uint64_t;
-  struct Regs {
-    Quad q;
-  };
-  __device__ static Regs Load(GlobalPtr<uint32_t> p, size_t g) { return {*(GlobalPtr<Quad>)(p + 4 * g)}; }
-  __device__ static Regs Load(LdsPtr<uint32_t> p, size_t g) { return {*(LdsPtr<Quad>)(p + 4 * g)}; }
-  __device__ static void Store(GlobalPtr<uint32_t> p, size_t g, const Regs& r) { *(GlobalPtr<Quad>)(p + 4 * g) = r.q; }
-  __device__ static void Store(LdsPtr<uint32_t> p, size_t g, const Regs& r) { *(LdsPtr<Quad>)(p + 4 * g) = r.q; }
-  template <int I>
-  __device__ static Word Get(const Regs& r) {
-    return static_cast<uint64_t>(r.q[2 * I]) | static_cast<uint64_t>(r.q[2 * I + 1]) << 32;
-  }
-  template <int I>
-  __device__ static void Set(Regs& r, Word w) {
-    r.q[2 * I] = static_cast<uint32_t>(w);
-    r.q[2 * I + 1] = static_cast<uint32_t>(w >> 32);
-  }
-  // inc added to the words whose bit i is set in `bits` (bits < 4)
-  __device__ static void Add(Regs& r, uint32_t bits, Inc inc) {
-    Set<0>(r, Get<0>(r) + (inc & (Word(0) - (bits & 1u))));
-    Set<1>(r, Get<1>(r) + (inc & (Word(0) - ((bits >> 1) & 1u))));
-  }
-  __device__ static Regs Pack(const Word* w) {
-    Regs r;
-    Set<0>(r, w[0]);
-    Set<1>(r, w[1]);
-    return r;
-  }
-};
-
 // k = 2: four 24-bit fields f = 2 e + label + 2^23 (e = G1 - G0) in 12 bytes (one global_load_dwordx3), the 96-bit
 // little-endian value f0 + f1 2^24 + f2 2^48 + f3 2^72. |2 e + label| <= 2 deg + 1 < 2^21 for n <= 2^20, so every
 // field stays inside (0, 2^24) and adding to one field never carries into the next: a group's neighbour update is
@@ -992,16 +956,15 @@ struct GroupK2 {
   }
 };
 
-// k = 3 when every degree is below 2^15: 48-bit words G0 | G1 << 15 | G2 << 30 | label << 45, two in 12 bytes (the
-// 96-bit little-endian value w0 + w1 2^48). A neighbour update adds (1 << 15 to) - (1 << 15 from) to its word as a
-// signed 96-bit add; as with the 64-bit words no field borrows or carries (G[from] >= 1, G[to] + 1 <= deg < 2^15).
-struct GroupK3N {
+// k = 3: two 48-bit words in 12 bytes, the 96-bit little-endian value w0 + w1 2^48. A neighbour update adds a signed
+// increment to its word; no field of a word borrows or carries, so this is one 96-bit add per group. Two word
+// formats (Words: K3Counts, K3Diffs) share the layout.
+template <typename Words>
+struct GroupK3 {
   static constexpr int kV = 2;
   static constexpr int kBytes = 12;
-  static constexpr int kBits = 15;
-  static constexpr int kLabelShift = 45;
   using Word = uint64_t;
-  using Inc = uint64_t;  // (1 << 15 to) - (1 << 15 from), two's complement
+  using Inc = typename Words::Inc;
   using Regs = GroupK2::Regs;
   __device__ static Regs Load(GlobalPtr<uint32_t> p, size_t g) { return GroupK2::Load(p, g); }
   __device__ static Regs Load(LdsPtr<uint32_t> p, size_t g) { return GroupK2::Load(p, g); }
@@ -1022,12 +985,13 @@ struct GroupK3N {
       r.d2 = static_cast<uint32_t>(w >> 16);
     }
   }
-  __device__ static void Add(Regs& r, uint32_t bits, Inc inc) {
-    const __int128 i = static_cast<int64_t>(inc);
-    const __int128 a = ((bits & 1u) ? i : __int128(0)) + ((bits & 2u) ? i * (__int128(1) << 48) : __int128(0));
+  // inc to the words whose bit i is set in `bits` (bits < 4)
+  __device__ static void Add(Regs& r, uint32_t bits, const Inc& inc) {
+    const __int128 i0 = (bits & 1u) ? Words::Increment(inc, (r.d1 >> (Words::kLabelShift - 32)) & 3u) : 0;
+    const __int128 i1 = (bits & 2u) ? Words::Increment(inc, (r.d2 >> (Words::kLabelShift - 16)) & 3u) : 0;
     unsigned __int128 v = static_cast<unsigned __int128>(r.d0) | static_cast<unsigned __int128>(r.d1) << 32 |
                           static_cast<unsigned __int128>(r.d2) << 64;
-    v += static_cast<unsigned __int128>(a);
+    v += static_cast<unsigned __int128>(i0 + i1 * (__int128(1) << 48));
     r.d0 = static_cast<uint32_t>(v);
     r.d1 = static_cast<uint32_t>(v >> 32);
     r.d2 = static_cast<uint32_t>(v >> 64);
@@ -1036,6 +1000,120 @@ struct GroupK3N {
     return {static_cast<uint32_t>(w[0]),
             (static_cast<uint32_t>(w[0] >> 32) & 0xFFFFu) | static_cast<uint32_t>(w[1]) << 16,
             static_cast<uint32_t>(w[1] >> 16)};
+  }
+};
+
+// The two clusters other than a, c1 < c2.
+__device__ __forceinline__ int OtherLow3(int a) { return a == 0 ? 1 : 0; }
+__device__ __forceinline__ int OtherHigh3(int a) { return a == 2 ? 1 : 2; }
+
+// Best move of a vertex with label a from its two gains over the other clusters c1 < c2, e_c = s_c - s_a + 1 -
+// 2 (G_c - G_a) = delta(a -> c): the smaller, ties to c1.
+__device__ __forceinline__ void Best3(int a, int e1, int e2, int* d, int* to) {
+  const bool take2 = e2 < e1;
+  *d = take2 ? e2 : e1;
+  *to = take2 ? OtherHigh3(a) : OtherLow3(a);
+}
+
+// k = 3 words while every degree is below 2^15: G0 | G1 << 15 | G2 << 30 | label << 45. A neighbour's increment is
+// (1 << 15 to) - (1 << 15 from) whatever its label (G[from] >= 1, G[to] + 1 <= deg < 2^15).
+struct K3Counts {
+  static constexpr int kBits = 15;
+  static constexpr uint64_t kMask = (1ull << kBits) - 1;
+  static constexpr int kLabelShift = 45;
+  using Inc = int64_t;
+  __device__ static int64_t Increment(const Inc& inc, uint32_t) { return inc; }
+  __device__ static Inc IncrementFor(int from, int to) {
+    return static_cast<int64_t>((1ull << (kBits * to)) - (1ull << (kBits * from)));
+  }
+  __device__ static uint64_t Encode(int g0, int g1, int g2, int a) {
+    return static_cast<uint64_t>(g0) | static_cast<uint64_t>(g1) << kBits | static_cast<uint64_t>(g2) << (2 * kBits) |
+           static_cast<uint64_t>(a) << kLabelShift;
+  }
+  // the counts and the label of a word (the degree is not needed)
+  __device__ static int Decode(uint64_t w, int, int* g) {
+    g[0] = static_cast<int>(w & kMask);
+    g[1] = static_cast<int>((w >> kBits) & kMask);
+    g[2] = static_cast<int>((w >> (2 * kBits)) & kMask);
+    return static_cast<int>(w >> kLabelShift);
+  }
+  __device__ static void Best(uint64_t w, int s0, int s1, int s2, int* a, int* d, int* to) {
+    *a = static_cast<int>(w >> kLabelShift);
+    const int h0 = s0 - 2 * static_cast<int>(w & kMask);
+    const int h1 = s1 - 2 * static_cast<int>((w >> kBits) & kMask);
+    const int h2 = s2 - 2 * static_cast<int>((w >> (2 * kBits)) & kMask);
+    const int ha = *a == 0 ? h0 : (*a == 1 ? h1 : h2);
+    Best3(*a, (*a == 0 ? h1 : h0) - ha + 1, (*a == 2 ? h1 : h2) - ha + 1, d, to);
+  }
+  __device__ static uint64_t Relabel(uint64_t w, int, int to) {
+    return (w & ~(3ull << kLabelShift)) | static_cast<uint64_t>(to) << kLabelShift;
+  }
+};
+
+// k = 3 words for any degree: for a vertex with label a and the other clusters c1 < c2, the differences
+// x1 = G_c1 - G_a and x2 = G_c2 - G_a (|x| <= deg < 2^20) as 21-bit fields biased by 2^20, and a:
+// (x1 + 2^20) | (x2 + 2^20) << 21 | a << 42. delta(a -> c) = (s_c - s_a + 1) - 2 (G_c - G_a) needs only these. A
+// neighbour's increment depends on its own label, one of three values per move; G is restored at the end of a
+// descent with G_a = (deg - x1 - x2) / 3.
+struct K3Diffs {
+  static constexpr int kBits = 21;
+  static constexpr uint64_t kMask = (1ull << kBits) - 1;
+  static constexpr int kBias = 1 << 20;
+  static constexpr int kLabelShift = 42;
+  struct Inc {
+    int64_t by_label[3];
+  };
+  __device__ static int64_t Increment(const Inc& inc, uint32_t a) {
+    return a == 0 ? inc.by_label[0] : (a == 1 ? inc.by_label[1] : inc.by_label[2]);
+  }
+  // a move from -> to changes a neighbour's G by +1 at `to` and -1 at `from`, so its differences by
+  // dG[c] - dG[a] for the neighbour's label a
+  __device__ static Inc IncrementFor(int from, int to) {
+    Inc inc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int da = (a == to) - (a == from);
+      const int c1 = OtherLow3(a), c2 = OtherHigh3(a);
+      const int dx1 = (c1 == to) - (c1 == from) - da;
+      const int dx2 = (c2 == to) - (c2 == from) - da;
+      inc.by_label[a] = static_cast<int64_t>(dx1) + static_cast<int64_t>(dx2) * (int64_t(1) << kBits);
+    }
+    return inc;
+  }
+  // the word of a vertex with label a and counts (or counts less any common offset) g0, g1, g2
+  __device__ static uint64_t Encode(int g0, int g1, int g2, int a) {
+    const int ga = a == 0 ? g0 : (a == 1 ? g1 : g2);
+    const int x1 = (a == 0 ? g1 : g0) - ga;
+    const int x2 = (a == 2 ? g1 : g2) - ga;
+    return static_cast<uint64_t>(x1 + kBias) | static_cast<uint64_t>(x2 + kBias) << kBits |
+           static_cast<uint64_t>(a) << kLabelShift;
+  }
+  __device__ static int Decode(uint64_t w, int deg, int* g) {
+    const int a = static_cast<int>(w >> kLabelShift);
+    const int x1 = static_cast<int>(w & kMask) - kBias;
+    const int x2 = static_cast<int>((w >> kBits) & kMask) - kBias;
+    const int ga = (deg - x1 - x2) / 3;
+    g[a] = ga;
+    g[OtherLow3(a)] = ga + x1;
+    g[OtherHigh3(a)] = ga + x2;
+    return a;
+  }
+  __device__ static void Best(uint64_t w, int s0, int s1, int s2, int* a, int* d, int* to) {
+    *a = static_cast<int>(w >> kLabelShift);
+    const int x1 = static_cast<int>(w & kMask) - kBias;
+    const int x2 = static_cast<int>((w >> kBits) & kMask) - kBias;
+    const int sa = *a == 0 ? s0 : (*a == 1 ? s1 : s2);
+    const int sc1 = *a == 0 ? s1 : s0;
+    const int sc2 = *a == 2 ? s1 : s2;
+    Best3(*a, sc1 - sa + 1 - 2 * x1, sc2 - sa + 1 - 2 * x2, d, to);
+  }
+  // the moved vertex's word for its new label: differences against G_to (its own G does not change)
+  __device__ static uint64_t Relabel(uint64_t w, int from, int to) {
+    int rel[3];
+    rel[from] = 0;
+    rel[OtherLow3(from)] = static_cast<int>(w & kMask) - kBias;
+    rel[OtherHigh3(from)] = static_cast<int>((w >> kBits) & kMask) - kBias;
+    return Encode(rel[0], rel[1], rel[2], to);
   }
 };
 
@@ -1055,7 +1133,7 @@ __device__ __forceinline__ void ForEachWord(const typename G::Regs& r, F&& f) {
 template <typename G>
 constexpr int kGroupTile = kWave * G::kV * kStreamLoads;
 
-// One pass of a streaming wide-path descent over a slot's vertex words after a move (G: GroupK2 or GroupK3). The row
+// One pass of a streaming wide-path descent over a slot's vertex words after a move (G: GroupK2 or GroupK3<>). The row
 // bits of a lane's group are bits (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tiles
 // t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile, so they hold no vertex >= n, and
 // resident <= kWaves * ResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its resident
@@ -1292,13 +1370,11 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   }
 }
 
-// Wide path, k = 3: one packed word per vertex (G: GroupK3, or GroupK3N when every degree is below 2^15) in the
-// run's work buffer, which is sized for GroupK3. `resident_tiles` counts GroupK3 tiles of the launch's LDS.
-template <int T, typename G>
+// Wide path, k = 3: one 48-bit word per vertex (GroupK3<Words>: K3Counts while every degree is below 2^15, else
+// K3Diffs) in the run's work buffer.
+template <int T, typename Words>
 __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int resident_tiles) {
-  constexpr int kBits = G::kBits;
-  constexpr uint64_t kMask = (1ull << kBits) - 1;
-  constexpr int kLabelShift = G::kLabelShift;
+  using G = GroupK3<Words>;
   Ctl* ctl = r.ctl;
   constexpr int kWaves = T / kWave;
   const int n = r.n;
@@ -1317,10 +1393,8 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const int32_t* sizes_in = r.pop.sizes + static_cast<size_t>(src) * 3;
   const GlobalPtr<uint32_t> st =
       (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 3));
-  // the slot's first whole tiles in LDS (the launch provides resident_tiles GroupK3 tiles of it)
-  constexpr int kTileBytes = kWave * G::kBytes * kStreamLoads;
-  const int lds_tiles = resident_tiles * WideTileBytes(3) / kTileBytes;
-  const int resident = min(lds_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
+  // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
+  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
   const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
@@ -1329,18 +1403,10 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
 
   int best_d = 0, best_move = -1;
   auto consider = [&](uint64_t w, int u) {
-    const int a = static_cast<int>(w >> kLabelShift);
-    const int h0 = s0 - 2 * static_cast<int>(w & kMask);
-    const int h1 = s1 - 2 * static_cast<int>((w >> kBits) & kMask);
-    const int h2 = s2 - 2 * static_cast<int>((w >> (2 * kBits)) & kMask);
-    const int ha = a == 0 ? h0 : (a == 1 ? h1 : h2);
-    const int lo = a == 0 ? h1 : h0;  // the other cluster with the smaller index
-    const int hi = a == 2 ? h1 : h2;
-    const bool take_hi = hi < lo;
-    const int d = (take_hi ? hi : lo) - ha + 1;
+    int a, d, to;
+    Words::Best(w, s0, s1, s2, &a, &d, &to);
     if (d < best_d) {
       best_d = d;
-      const int to = take_hi ? (a == 2 ? 1 : 2) : (a == 0 ? 1 : 0);
       best_move = (u << 4) | (a << 2) | to;
     }
   };
@@ -1352,10 +1418,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int u = u0 + i;
-      w[i] = u < n ? static_cast<uint64_t>(g_in[u]) | static_cast<uint64_t>(g_in[n + u]) << kBits |
-                         static_cast<uint64_t>(g_in[2 * n + u]) << (2 * kBits) |
-                         static_cast<uint64_t>(lab_in[u]) << kLabelShift
-                   : 0;
+      w[i] = u < n ? Words::Encode(g_in[u], g_in[n + u], g_in[2 * n + u], lab_in[u]) : 0;
       if (u < n) consider(w[i], u);
     }
     if (g < res_groups) {
@@ -1385,19 +1448,16 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
     s0 += (to == 0) - (from == 0);
     s1 += (to == 1) - (from == 1);
     s2 += (to == 2) - (from == 2);
-    const uint64_t inc = (1ull << (kBits * to)) - (1ull << (kBits * from));
-    const uint64_t label_to = static_cast<uint64_t>(to) << kLabelShift;
     const GlobalWords32 row = (GlobalWords32)(row_base + static_cast<size_t>(v) * row_words);
     best_d = 0;
     best_move = -1;
     StreamMove<T, G>(
-        st, res, resident, row, n, wave, lane, inc, v,
-        [label_to](typename G::Regs& x, int i) {
-          auto relabel = [label_to](uint64_t w) { return (w & ~(3ull << kLabelShift)) | label_to; };
+        st, res, resident, row, n, wave, lane, Words::IncrementFor(from, to), v,
+        [from, to](typename G::Regs& x, int i) {
           if (i == 0) {
-            G::template Set<0>(x, relabel(G::template Get<0>(x)));
+            G::template Set<0>(x, Words::Relabel(G::template Get<0>(x), from, to));
           } else {
-            G::template Set<1>(x, relabel(G::template Get<1>(x)));
+            G::template Set<1>(x, Words::Relabel(G::template Get<1>(x), from, to));
           }
         },
         consider);
@@ -1419,10 +1479,11 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
     ForEachWord<G>(x, [&](uint64_t w, int i) {
       const int u = u0 + i;
       if (u < n) {
-        g_out[u] = static_cast<int32_t>(w & kMask);
-        g_out[n + u] = static_cast<int32_t>((w >> kBits) & kMask);
-        g_out[2 * n + u] = static_cast<int32_t>((w >> (2 * kBits)) & kMask);
-        lab_out[u] = static_cast<uint8_t>(w >> kLabelShift);
+        int gc[3];
+        lab_out[u] = static_cast<uint8_t>(Words::Decode(w, r.deg[u], gc));
+        g_out[u] = gc[0];
+        g_out[n + u] = gc[1];
+        g_out[2 * n + u] = gc[2];
       }
     });
   });
@@ -1755,10 +1816,10 @@ __device__ __forceinline__ int BlockMaxDegree(const RunDev& r) {
 template <int T>
 __global__ void __launch_bounds__(T) KernelDescentK3Wide(const RunDev* runs, ItemList list, int resident_tiles) {
   Persistent(runs, list, [resident_tiles](const RunDev& r, int s) {
-    if (BlockMaxDegree<T>(r) < (1 << GroupK3N::kBits)) {
-      DescentK3WideBody<T, GroupK3N>(r, s, resident_tiles);
+    if (BlockMaxDegree<T>(r) < (1 << K3Counts::kBits)) {
+      DescentK3WideBody<T, K3Counts>(r, s, resident_tiles);
     } else {
-      DescentK3WideBody<T, GroupK3>(r, s, resident_tiles);
+      DescentK3WideBody<T, K3Diffs>(r, s, resident_tiles);
     }
   });
 }

@@ -64,7 +64,8 @@ struct RunDev {
   uint64_t* masks;       // [P][k][words] rebuild scratch
   int64_t* intra;        // [P] rebuild scratch: sum_v G[label v][v]
   uint64_t* work;        // packed vertex words of the wide descents (null for other plans): k = 3 chunk kernel
-                         // [P][WideStride(n, 3)] u64; streaming kernels [P][WideSlotBytes(n, k)] bytes
+                         // [P][WideStride(n, 3)] u64; streaming kernels [P][WideSlotBytes(n, k)] bytes (k = 3:
+                         // within the chunk kernel's size, so a footprint needs no edge count)
   uint64_t init_stream, select_stream, perturb_stream;
   int32_t tournament, copies, early_stop, iterations;
   float perturbation;
@@ -101,12 +102,12 @@ inline const char* DeviceErrorText(int32_t error) {
 }
 
 // The streaming wide-path descents load and store a group of consecutive vertices per lane: k = 2 four 24-bit fields
-// in 12 bytes, k = 3 two 64-bit words in 16 bytes. A wave's tile is kStreamLoads such groups of its 64 lanes (1024
-// vertices in 3 KiB at k = 2, 512 in 4 KiB at k = 3); each slot is padded to whole tiles (WideSlotBytes per slot in
+// in 12 bytes, k = 3 two 48-bit words in 12 bytes. A wave's tile is kStreamLoads such groups of its 64 lanes (1024
+// vertices at k = 2, 512 at k = 3, 3 KiB either way); each slot is padded to whole tiles (WideSlotBytes per slot in
 // RunDev::work, 16-byte aligned).
 constexpr int kStreamLoads = 4;
 __host__ __device__ constexpr int WideGroupVertices(int k) { return k == 2 ? 4 : 2; }
-__host__ __device__ constexpr int WideGroupBytes(int k) { return k == 2 ? 12 : 16; }
+__host__ __device__ constexpr int WideGroupBytes(int) { return 12; }
 __host__ __device__ constexpr int WideTileVertices(int k) { return 64 * WideGroupVertices(k) * kStreamLoads; }
 __host__ __device__ constexpr int WideTileBytes(int k) { return 64 * WideGroupBytes(k) * kStreamLoads; }
 // n rounded up to whole tiles

@@ -423,7 +423,7 @@ void LayoutRun(const RunSpec& s, RunDev& r, Arena& a) {
   a.Take(r.intra, P);
   size_t work = 0;  // in 64-bit words; the same for both k = 3 kernels, so a footprint needs no edge count
   if (s.plan.descent == Descent::kK3Wide || s.plan.descent == Descent::kK3Chunk) {
-    work = P * static_cast<size_t>(WideSlotBytes(static_cast<int>(n), 3)) / 8;
+    work = P * static_cast<size_t>(WideStride(static_cast<int>(n), 3));  // the chunk kernel's u64 per vertex
   }
   if (s.plan.descent == Descent::kK2Wide) work = P * static_cast<size_t>(WideSlotBytes(static_cast<int>(n), 2)) / 8;
   a.Take(r.work, work);

@@ -301,19 +301,18 @@ def dense_wide_graph(cga):
 
 # (k, threads, blocks, tiles in LDS): None = the default plan's (every whole tile that fits, at most two per wave at
 # k = 2 and three at k = 3); k = 2 at 1024 threads keeps 32 of the 33 tiles of n = 33001 (the partial last one is
-# streamed), k = 3 keeps 39 of 65 (the LDS limit: three for waves 0-6, two for the others); 16 tiles of LDS at 256
-# threads are capped at 12 (three per wave); 5 tiles leave waves 5-15 without one; 0 streams every tile.
+# streamed), k = 3 keeps 48 of 65; 16 tiles of LDS at 256 threads are capped at 12 (three per wave); 5 tiles leave
+# waves 5-15 without one; 0 streams every tile.
 @pytest.mark.parametrize("k,threads,blocks,tiles", [(2, 1024, 0, None), (3, 1024, 0, None), (2, 256, 0, 0),
                                                     (3, 256, 2, 16), (2, 1024, 0, 5), (3, 1024, 0, 0)])
 def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dense_wide_graph, k, threads, blocks,
                                            tiles):
-    """The streaming wide kernels (per lane a group of four 24-bit vertex fields in 12 bytes at k = 2, two 64-bit
-    words in 16 bytes at k = 3, one row dword per group) on a dense graph, where auto takes them for k = 3 too, with
-    the slot's first tiles (3 KiB at k = 2, 4 KiB at k = 3) kept in LDS; each workgroup size, fewer resident
-    workgroups than slots, and fewer or no tiles in LDS are forced through a tuned-plan entry. The trajectory must
-    be the oracle's."""
+    """The streaming wide kernels (per lane a group of 12 bytes: four 24-bit vertex fields at k = 2, two 48-bit words
+    at k = 3; one row dword per group) on a dense graph, where auto takes them for k = 3 too, with the slot's first
+    3 KiB tiles kept in LDS; each workgroup size, fewer resident workgroups than slots, and fewer or no tiles in LDS
+    are forced through a tuned-plan entry. The trajectory must be the oracle's."""
     g = dense_wide_graph
-    tile_bytes = 3072 if k == 2 else 4096
+    tile_bytes = 3072
     kw = dict(k=k, population=4, iterations=2, early_stop=2, seed=5, gww=0.25)
     if tiles is None:
         got, want = run_both(cga, orc, g, cpu_threads, **kw)
@@ -329,10 +328,10 @@ def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dens
 
 
 def test_k3_streaming_words_follow_the_largest_degree(cga, orc, cpu_threads):
-    """The k = 3 streaming kernel packs 48-bit words (15-bit counts, two vertices in 12 bytes) while every degree is
-    below 2^15 and 64-bit words otherwise. An ER graph (n = 40000, p = 0.05: dense enough for the streaming kernel)
-    takes the 48-bit words; the same graph with vertex 0 joined to every other vertex (degree 39999) takes the 64-bit
-    words. Both must follow the oracle."""
+    """The k = 3 streaming kernel's 48-bit words hold the three counts (15 bits each) while every degree is below
+    2^15, and otherwise the label and the two differences G_c - G_a (21 bits each). An ER graph (n = 40000, p = 0.05:
+    dense enough for the streaming kernel) takes the counts; the same graph with vertex 0 joined to every other
+    vertex (degree 39999) takes the differences. Both must follow the oracle."""
     base = cga.erdos_renyi(40000, 0.05, 13)
     n = base.n
     bits = base.bits.copy()
