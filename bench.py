@@ -80,10 +80,10 @@ REGISTER_DESCENTS = ("registers", "k2-registers", "k3-packed", "generic-lds")
 
 def streamed_state_bytes(n, k, lds_bytes):
     """State bytes a streaming wide descent passes over per move: one packed word per vertex (k = 2: 3 bytes; k = 3:
-    6 bytes) less the whole 3 KiB tiles it keeps in LDS (1024 vertices at k = 2, at most two per wave; 512 at k = 3,
-    at most three per wave; 1024 threads)."""
+    6 bytes) less the whole 3 KiB tiles it keeps in LDS (1024 vertices at k = 2, 512 at k = 3; at most three per
+    wave of 1024 threads)."""
     if k == 2:
-        return 3 * n - 3072 * min(lds_bytes // 3072, n // 1024, 32)
+        return 3 * n - 3072 * min(lds_bytes // 3072, n // 1024, 48)
     return 6 * n - 3072 * min(lds_bytes // 3072, n // 512, 48)
 
 

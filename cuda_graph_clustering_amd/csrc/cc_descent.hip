@@ -1136,7 +1136,7 @@ constexpr int kGroupTile = kWave * G::kV * kStreamLoads;
 // One pass of a streaming wide-path descent over a slot's vertex words after a move (G: GroupK2 or GroupK3<>). The row
 // bits of a lane's group are bits (l kV) % 32 + i of a single row dword (32 / kV lanes read the same one). Tiles
 // t < `resident` live in LDS (`res`, the same layout; resident <= n / kTile, so they hold no vertex >= n, and
-// resident <= kWaves * ResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its resident
+// resident <= kWaves * kResidentPerWave), the others in HBM (`st`). A wave first issues the row dwords of its resident
 // tiles and the loads of its first streamed tile, then updates its resident tiles while those loads are in flight,
 // then streams the rest, loading its next tile before it consumes the current one (two tiles of groups and row dwords
 // per lane in flight behind the arithmetic). Every word of a neighbour of the moved vertex gets `inc`,
@@ -1154,7 +1154,7 @@ __device__ __forceinline__ void StreamMove(GlobalPtr<uint32_t> st, LdsPtr<uint32
   constexpr int kJ = kStreamLoads;
   constexpr int kTile = kGroupTile<G>;
   constexpr int kWaves = T / kWave;
-  constexpr int kRes = ResidentPerWave(kV == 4 ? 2 : 3);
+  constexpr int kRes = kResidentPerWave;
   const int tiles = (n + kTile - 1) / kTile;
   const uint32_t lane_bit = static_cast<uint32_t>(lane * kV) & 31u;
   const int lane_dword = (lane * kV) >> 5;
@@ -1273,7 +1273,7 @@ __device__ __forceinline__ void DescentK2WideBody(const RunDev& r, int s, int re
   const GlobalPtr<uint32_t> st =
       (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 2));
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(2)));
+  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * kResidentPerWave));
   const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
@@ -1394,7 +1394,7 @@ __device__ __forceinline__ void DescentK3WideBody(const RunDev& r, int s, int re
   const GlobalPtr<uint32_t> st =
       (GlobalPtr<uint32_t>)(reinterpret_cast<uint8_t*>(r.work) + static_cast<size_t>(s) * WideSlotBytes(n, 3));
   // the slot's first whole tiles in LDS (the launch provides resident_tiles tiles of it)
-  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * ResidentPerWave(3)));
+  const int resident = min(resident_tiles, min(n / kGroupTile<G>, kWaves * kResidentPerWave));
   const size_t res_groups = static_cast<size_t>(resident) * (kGroupTile<G> / G::kV);
   extern __shared__ __align__(16) uint8_t dyn[];
   const LdsPtr<uint32_t> res = (LdsPtr<uint32_t>)dyn;
@@ -2180,7 +2180,7 @@ size_t GenericLdsBytes(int n, int k, int wide) {
 size_t WideResidentLdsBytes(int n, int k, int threads) {
   const int64_t tiles = n / WideTileVertices(k);  // whole tiles only
   const int64_t fit = static_cast<int64_t>(DescentLdsLimit() / WideTileBytes(k));
-  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * ResidentPerWave(k);
+  const int64_t per_waves = static_cast<int64_t>(threads / kWave) * kResidentPerWave;
   return static_cast<size_t>(std::min({tiles, fit, per_waves})) * WideTileBytes(k);
 }
 

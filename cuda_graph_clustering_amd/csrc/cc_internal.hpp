@@ -117,10 +117,10 @@ __host__ __device__ constexpr int64_t WideStride(int n, int k) {
 __host__ __device__ constexpr int64_t WideSlotBytes(int n, int k) {
   return WideStride(n, k) / WideGroupVertices(k) * WideGroupBytes(k);
 }
-// The first tiles of a slot may live in the workgroup's LDS instead of HBM (at most ResidentPerWave per wave, as
+// The first tiles of a slot may live in the workgroup's LDS instead of HBM (at most kResidentPerWave per wave, as
 // many as the LDS holds): they are never streamed. A resident tile holds its row dwords in registers through the
-// move; three of them pushed the k = 2 kernel past 128 VGPRs at 1024 threads.
-__host__ __device__ constexpr int ResidentPerWave(int k) { return k == 3 ? 3 : 2; }
+// move (four VGPRs per tile).
+constexpr int kResidentPerWave = 3;
 
 // ---- population kernels (cc_kernels.hip); max_n is the largest n among the runs of the items -------------------
 
@@ -202,7 +202,7 @@ size_t GenericLdsBytes(int n, int k, int wide);
 // the dirty-chunk list of each move parity (2 x u32 per chunk).
 __host__ __device__ constexpr size_t K3ChunkLdsBytes(int n) { return (static_cast<size_t>(n) + 63) / 64 * (6 + 2) * 4; }
 // LDS of the streaming wide-path descents (k = 2, k = 3) with `threads` per workgroup: the slot's first tiles, as
-// many as fit (at most ResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
+// many as fit (at most kResidentPerWave per wave and no more than the slot's whole tiles; a partial last tile is
 // always streamed).
 size_t WideResidentLdsBytes(int n, int k, int threads);
 int RegistersVpt(int n, int k, int* threads);

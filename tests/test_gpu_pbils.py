@@ -299,8 +299,8 @@ def dense_wide_graph(cga):
     return cga.erdos_renyi(33001, 0.12, 21)
 
 
-# (k, threads, blocks, tiles in LDS): None = the default plan's (every whole tile that fits, at most two per wave at
-# k = 2 and three at k = 3); k = 2 at 1024 threads keeps 32 of the 33 tiles of n = 33001 (the partial last one is
+# (k, threads, blocks, tiles in LDS): None = the default plan's (every whole tile that fits, at most three per
+# wave); k = 2 at 1024 threads keeps 32 of the 33 tiles of n = 33001 (the partial last one is
 # streamed), k = 3 keeps 48 of 65; 16 tiles of LDS at 256 threads are capped at 12 (three per wave); 5 tiles leave
 # waves 5-15 without one; 0 streams every tile.
 @pytest.mark.parametrize("k,threads,blocks,tiles", [(2, 1024, 0, None), (3, 1024, 0, None), (2, 256, 0, 0),
@@ -317,7 +317,7 @@ def test_dense_wide_path_streaming_kernels(cga, orc, cpu_threads, tmp_path, dens
     if tiles is None:
         got, want = run_both(cga, orc, g, cpu_threads, **kw)
         tiles = min(g.n // (1024 if k == 2 else 512), (160 * 1024 - 2048) // tile_bytes,
-                    threads // 64 * (2 if k == 2 else 3))
+                    threads // 64 * 3)
     else:
         descent = 3 if k == 2 else 4
         got, want = run_with_plan(cga, orc, g, cpu_threads, tmp_path,
