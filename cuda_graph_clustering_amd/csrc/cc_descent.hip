@@ -986,15 +986,20 @@ struct GroupK3 {
     }
   }
   // inc to the words whose bit i is set in `bits` (bits < 4)
+  // Two carry chains of 32-bit adds: word 0's increment sign-extended over the three dwords, then word 1's
+  // (shifted by 48 bits: dwords 1 and 2 only). Increments are below 2^31 in magnitude.
   __device__ static void Add(Regs& r, uint32_t bits, const Inc& inc) {
-    const __int128 i0 = (bits & 1u) ? Words::Increment(inc, (r.d1 >> (Words::kLabelShift - 32)) & 3u) : 0;
-    const __int128 i1 = (bits & 2u) ? Words::Increment(inc, (r.d2 >> (Words::kLabelShift - 16)) & 3u) : 0;
-    unsigned __int128 v = static_cast<unsigned __int128>(r.d0) | static_cast<unsigned __int128>(r.d1) << 32 |
-                          static_cast<unsigned __int128>(r.d2) << 64;
-    v += static_cast<unsigned __int128>(i0 + i1 * (__int128(1) << 48));
-    r.d0 = static_cast<uint32_t>(v);
-    r.d1 = static_cast<uint32_t>(v >> 32);
-    r.d2 = static_cast<uint32_t>(v >> 64);
+    const int32_t i0 = static_cast<int32_t>(Words::Increment(inc, (r.d1 >> (Words::kLabelShift - 32)) & 3u)) &
+                       -static_cast<int32_t>(bits & 1u);
+    const int32_t i1 = static_cast<int32_t>(Words::Increment(inc, (r.d2 >> (Words::kLabelShift - 16)) & 3u)) &
+                       -static_cast<int32_t>((bits >> 1) & 1u);
+    const uint32_t x0 = static_cast<uint32_t>(i0 >> 31);
+    unsigned c;
+    r.d0 = __builtin_addc(r.d0, static_cast<uint32_t>(i0), 0u, &c);
+    r.d1 = __builtin_addc(r.d1, x0, c, &c);
+    r.d2 = __builtin_addc(r.d2, x0, c, &c);
+    r.d1 = __builtin_addc(r.d1, static_cast<uint32_t>(i1) << 16, 0u, &c);
+    r.d2 = __builtin_addc(r.d2, static_cast<uint32_t>(i1 >> 16), c, &c);
   }
   __device__ static Regs Pack(const Word* w) {
     return {static_cast<uint32_t>(w[0]),
